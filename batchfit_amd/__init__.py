@@ -1,0 +1,184 @@
+"""Batched bounded Levenberg–Marquardt fitting of Gaussian, Lorentzian,
+pseudo-Voigt and paired-Lorentzian spectra on AMD Instinct MI355X (gfx950).
+
+``fit`` keeps the argument meaning, validation and result types of the
+reference's ``batchfit.fit`` (python/batchfit/__init__.py:63-112) and adds the
+device ``"hip"``, which runs hand-written CDNA4 kernels through the C ABI of
+``libbatchfit_hip.so`` (include/batchfit_hip.h). Like the reference's explicit
+GPU device, ``"hip"`` computes in float32 and returns float32 arrays
+(python/README.md:56-59). The reference's CPU device is not part of this
+package: ``device="cpu"`` is rejected with a message rather than served by a
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from enum import IntEnum
+from typing import NamedTuple
+
+import numpy as np
+
+from . import _native
+
+__version__ = "0.1.0"
+__all__ = ["fit", "FitResult", "FitResultWithUncertainties", "FitStatus", "UncertaintyStatus",
+           "hip_available", "NUMERICS_DEFAULT", "__version__"]
+
+_PARAMETERS = dict(_native.PARAMETERS)
+
+# Compensated FP32 partials + FP64 combine/solve/accept/covariance.
+NUMERICS_DEFAULT = _native.COMPENSATED | _native.MIXED_FP64
+
+
+class FitStatus(IntEnum):
+    """cpu/solver.hpp:16-23."""
+    converged = 0
+    max_iterations = 1
+    singular_matrix = 2
+    invalid_input = 3
+    degenerate_data = 4
+    unresolved_peak = 5
+
+
+class UncertaintyStatus(IntEnum):
+    """uncertainty_types.hpp:3-10."""
+    available = 0
+    not_converged = 1
+    insufficient_dof = 2
+    singular_matrix = 3
+    active_bound = 4
+    invalid_input = 5
+
+
+class FitResult(NamedTuple):
+    params: np.ndarray
+    status: np.ndarray
+    iterations: np.ndarray
+    chi2: np.ndarray
+
+
+class FitResultWithUncertainties(NamedTuple):
+    params: np.ndarray
+    status: np.ndarray
+    iterations: np.ndarray
+    chi2: np.ndarray
+    covariance: np.ndarray
+    stderr: np.ndarray
+    uncertainty_status: np.ndarray
+
+
+def hip_available() -> bool:
+    """Whether the library is built and a gfx950 device is usable."""
+    try:
+        return bool(_native.library().bf_hip_available())
+    except ImportError:
+        return False
+
+
+def _real_array(value, name):
+    array = np.asarray(value)
+    if array.dtype.kind not in "fiu":
+        raise TypeError(f"{name} must contain real numeric values")
+    return array
+
+
+_MAPPINGS = {"auto": (0, 0), "lane": (1, 0), "group16": (2, 16), "group32": (2, 32),
+             "group64": (2, 64), "wave": (2, 64)}
+
+
+def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncertainties=False,
+        numerics=None, mapping="auto", max_iterations=100):
+    """Fit shared x[M] and y[N,M]; return params[N,P], status/iterations/chi2[N].
+
+    A single y[M] is accepted as N=1. ``device`` is "auto" or "hip"; both run
+    on the GPU in float32 and never fall back to a CPU path. ``p0=None`` runs
+    the data-only initial guess on the device. ``uncertainties=True`` adds
+    covariance[N,P,P], stderr[N,P] and uncertainty_status[N].
+
+    Keyword-only extensions of the HIP device: ``numerics`` (bit0 compensated
+    partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
+    ``mapping`` ("auto", "lane", "group16", "group32", "group64") and
+    ``max_iterations`` (default 100, cpu/solver.hpp:27).
+    """
+    if model not in _PARAMETERS:
+        raise ValueError(f"unknown model {model!r}; choose one of {tuple(_PARAMETERS)}")
+    if device == "cpu":
+        raise ValueError("device 'cpu' is the reference CPU path, which this package does not "
+                         "include; use device='hip'")
+    if device not in ("auto", "hip"):
+        raise ValueError("device must be 'auto' or 'hip'")
+    if not isinstance(uncertainties, (bool, np.bool_)):
+        raise TypeError("uncertainties must be a boolean")
+    if mapping not in _MAPPINGS:
+        raise ValueError(f"mapping must be one of {tuple(_MAPPINGS)}")
+    x, y = _real_array(x, "x"), _real_array(y, "y")
+    if x.ndim != 1 or not x.size:
+        raise ValueError("x must have nonempty shape [M]")
+    if y.ndim == 1:
+        y = y.reshape(1, -1)
+    if y.ndim != 2 or y.shape[1] != x.size:
+        raise ValueError("y must have shape [N,M] matching x[M]")
+    dtype = np.dtype(np.float32)
+    x = np.require(x, dtype=dtype, requirements=["C", "A"])
+    y = np.require(y, dtype=dtype, requirements=["C", "A"])
+    n, p = y.shape[0], _PARAMETERS[model]
+    if n >= 2**32:
+        raise ValueError("at most 2**32-1 fits per call")
+    if p0 is not None:
+        p0 = _real_array(p0, "p0")
+        if p0.shape == (p,):
+            p0 = np.broadcast_to(p0, (n, p))
+        if p0.shape != (n, p):
+            raise ValueError(f"p0 must have shape [{p}] or [{n},{p}]")
+        p0 = np.require(p0, dtype=dtype, requirements=["C", "A"])
+    if bounds is None:
+        lower, upper = np.full(p, -np.inf, dtype=dtype), np.full(p, np.inf, dtype=dtype)
+    else:
+        if not isinstance(bounds, (tuple, list)) or len(bounds) != 2:
+            raise ValueError("bounds must be (lower, upper), each scalar or [P]")
+        lower, upper = (_real_array(value, "bounds") for value in bounds)
+        if lower.shape not in ((), (p,)) or upper.shape not in ((), (p,)):
+            raise ValueError(f"each bound must be scalar or have shape [{p}]")
+        maximum = np.finfo(np.float32).max
+        if any(np.any(np.isfinite(bound) & (np.abs(bound) > maximum)) for bound in (lower, upper)):
+            raise ValueError("finite bounds must be representable in the fitting dtype")
+        lower = np.require(np.broadcast_to(lower, (p,)), dtype=dtype, requirements=["C", "A"])
+        upper = np.require(np.broadcast_to(upper, (p,)), dtype=dtype, requirements=["C", "A"])
+    # python/bindings.cpp:153-161: invalid bounds are an argument error.
+    if (np.any(np.isnan(lower)) or np.any(np.isnan(upper)) or np.any(lower > upper)
+            or np.any(lower == np.inf) or np.any(upper == -np.inf)):
+        raise ValueError("invalid parameter bounds")
+
+    lib = _native.library()
+    if not lib.bf_hip_available():
+        raise RuntimeError("HIP fitting is unavailable in this build or on this device")
+    cfg = _native.default_config()
+    cfg.fits, cfg.points = n, x.size
+    cfg.max_iterations = int(max_iterations)
+    cfg.numerics = NUMERICS_DEFAULT if numerics is None else int(numerics)
+    for k in range(p):
+        cfg.lower[k], cfg.upper[k] = float(lower[k]), float(upper[k])
+    launch = _native.Launch()
+    launch.mapping, launch.group_lanes = _MAPPINGS[mapping]
+
+    params = np.empty((n, p), dtype)
+    status = np.empty(n, np.int32)
+    iterations = np.empty(n, np.uint32)
+    chi2 = np.empty(n, dtype)
+    cov = se = ustatus = None
+    if uncertainties:
+        cov = np.empty((n, p, p), dtype)
+        se = np.empty((n, p), dtype)
+        ustatus = np.empty(n, np.int32)
+
+    def ptr(a):
+        return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+    err = _native.error_buffer()
+    rc = lib.bf_hip_fit(_native.MODELS[model], ctypes.byref(cfg), ctypes.byref(launch), ptr(x),
+                        ptr(y), ptr(p0), ptr(params), ptr(status), ptr(iterations), ptr(chi2),
+                        ptr(cov), ptr(se), ptr(ustatus), err, len(err))
+    _native.check(rc, err)
+    if uncertainties:
+        return FitResultWithUncertainties(params, status, iterations, chi2, cov, se, ustatus)
+    return FitResult(params, status, iterations, chi2)

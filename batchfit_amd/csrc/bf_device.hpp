@@ -1,0 +1,287 @@
+// bf_device.hpp — CDNA4 device helpers for the batched LM fitter: finiteness
+// tests on bit patterns, the four peak models with fused value + analytic
+// Jacobian in FP32, and cross-lane sums for groups of 8/16/32/64 lanes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bfhip {
+
+enum ModelId : int { GAUSSIAN = 0, LORENTZIAN = 1, PSEUDO_VOIGT = 2, TWO_LORENTZIAN = 3 };
+template <int MODEL> struct Params {
+    static constexpr int P = MODEL == PSEUDO_VOIGT ? 4 : MODEL == TWO_LORENTZIAN ? 8 : 3;
+    static constexpr int TRI = P * (P + 1) / 2;  // packed lower triangle of JᵀJ
+    static constexpr int TERMS = TRI + P + 1;    // JᵀJ, Jᵀr, Σr²
+};
+
+// Status codes: cpu/solver.hpp:16-23 and uncertainty_types.hpp:3-10.
+enum : int32_t {
+    ST_CONVERGED = 0,
+    ST_MAX_ITERATIONS = 1,
+    ST_SINGULAR = 2,
+    ST_INVALID = 3,
+    ST_DEGENERATE = 4,
+    ST_UNRESOLVED = 5
+};
+enum : int32_t {
+    US_AVAILABLE = 0,
+    US_NOT_CONVERGED = 1,
+    US_DOF = 2,
+    US_SINGULAR = 3,
+    US_ACTIVE_BOUND = 4,
+    US_INVALID = 5
+};
+
+// Finiteness from the exponent bits: immune to any fast-math assumption.
+__device__ __forceinline__ bool finite_f(float v) {
+    return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u;
+}
+__device__ __forceinline__ bool nan_f(float v) {
+    return (__float_as_uint(v) & 0x7fffffffu) > 0x7f800000u;
+}
+__device__ __forceinline__ bool finite_d(double v) {
+    return (uint32_t(__double_as_longlong(v) >> 32) & 0x7ff00000u) != 0x7ff00000u;
+}
+__device__ __forceinline__ float qnan_f() { return __uint_as_float(0x7fc00000u); }
+
+// ---------------------------------------------------------------------------
+// Peak models (models.hpp:78-185), FP32, value and Jacobian from one set of
+// transcendentals. Per-pass constants (reciprocal width, mixing complement …)
+// are hoisted into ModelConsts once per parameter vector.
+//   Gaussian     exp(-z²/2) = 2^(-z²·log2(e)/2)
+//   pseudo-Voigt exp(-4 ln2 z²) = 2^(-4 z²)   (exact rewrite of the exponent)
+template <int MODEL> struct ModelConsts;
+
+template <> struct ModelConsts<GAUSSIAN> {
+    float a, c, iw, aiw;
+    __device__ __forceinline__ void set(const float *p) {
+        a = p[0];
+        c = p[1];
+        iw = 1.0f / p[2];
+        aiw = a * iw;
+    }
+    // returns f; jac[0..2]
+    __device__ __forceinline__ float eval(float x, float *jac) const {
+        const float z = (x - c) * iw;
+        const float e = __builtin_amdgcn_exp2f(z * z * -0.72134752044448170368f);
+        const float j1 = aiw * e * z;
+        jac[0] = e;
+        jac[1] = j1;
+        jac[2] = j1 * z;
+        return a * e;
+    }
+    __device__ __forceinline__ float value(float x) const {
+        const float z = (x - c) * iw;
+        return a * __builtin_amdgcn_exp2f(z * z * -0.72134752044448170368f);
+    }
+};
+
+template <> struct ModelConsts<LORENTZIAN> {
+    float a, c, iw, k;
+    __device__ __forceinline__ void set(const float *p) {
+        a = p[0];
+        c = p[1];
+        iw = 1.0f / p[2];
+        k = 2.0f * a * iw;
+    }
+    __device__ __forceinline__ float eval(float x, float *jac) const {
+        const float z = (x - c) * iw;
+        const float s = __builtin_amdgcn_rcpf(__builtin_fmaf(z, z, 1.0f));
+        const float j1 = k * s * s * z;
+        jac[0] = s;
+        jac[1] = j1;
+        jac[2] = j1 * z;
+        return a * s;
+    }
+    __device__ __forceinline__ float value(float x) const {
+        const float z = (x - c) * iw;
+        return a * __builtin_amdgcn_rcpf(__builtin_fmaf(z, z, 1.0f));
+    }
+};
+
+template <> struct ModelConsts<PSEUDO_VOIGT> {
+    float a, c, iw, eta, gmix, k;
+    __device__ __forceinline__ void set(const float *p) {
+        a = p[0];
+        c = p[1];
+        iw = 1.0f / p[2];
+        eta = p[3];
+        gmix = 1.0f - p[3];
+        k = 8.0f * a * iw;
+    }
+    __device__ __forceinline__ float eval(float x, float *jac) const {
+        const float z = (x - c) * iw;
+        const float z2 = z * z;
+        const float g = __builtin_amdgcn_exp2f(-4.0f * z2);
+        const float l = __builtin_amdgcn_rcpf(__builtin_fmaf(4.0f, z2, 1.0f));
+        const float shape = __builtin_fmaf(gmix, g, eta * l);
+        const float scale = k * __builtin_fmaf(gmix * 0.69314718055994530942f, g, eta * l * l);
+        const float j1 = scale * z;
+        jac[0] = shape;
+        jac[1] = j1;
+        jac[2] = j1 * z;
+        jac[3] = a * (l - g);
+        return a * shape;
+    }
+    __device__ __forceinline__ float value(float x) const {
+        const float z = (x - c) * iw;
+        const float z2 = z * z;
+        const float g = __builtin_amdgcn_exp2f(-4.0f * z2);
+        const float l = __builtin_amdgcn_rcpf(__builtin_fmaf(4.0f, z2, 1.0f));
+        return a * __builtin_fmaf(gmix, g, eta * l);
+    }
+};
+
+template <> struct ModelConsts<TWO_LORENTZIAN> {
+    ModelConsts<LORENTZIAN> first, second;
+    float b0, b1;
+    __device__ __forceinline__ void set(const float *p) {
+        first.set(p);
+        second.set(p + 3);
+        b0 = p[6];
+        b1 = p[7];
+    }
+    __device__ __forceinline__ float eval(float x, float *jac) const {
+        const float f1 = first.eval(x, jac);
+        const float f2 = second.eval(x, jac + 3);
+        jac[6] = 1.0f;
+        jac[7] = x;
+        return f1 + f2 + __builtin_fmaf(b1, x, b0);
+    }
+    __device__ __forceinline__ float value(float x) const {
+        return first.value(x) + second.value(x) + __builtin_fmaf(b1, x, b0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Cross-lane all-reduce within aligned groups of G lanes (G = 8, 16, 32, 64).
+// Steps: quad_perm xor1, quad_perm xor2, row_half_mirror (8), row_mirror (16),
+// ds_swizzle xor16 (32), permlane32_swap (64). Mirror partners of a lane lie
+// in the other half of its 8/16-lane block, so after each step every lane
+// holds the sum over the block.
+__device__ __forceinline__ int dpp_quad_xor1(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int dpp_quad_xor2(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int dpp_half_mirror(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int dpp_mirror(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int swz_xor16(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x401F); }
+// Partner across the two 32-lane halves, read through SGPRs: lanes 0-31 take
+// lane 32's value and lanes 32-63 take lane 0's. Only used after the 32-lane
+// steps, when every lane of a half already holds that half's result.
+__device__ __forceinline__ int perm_xor32(int v) {
+    const int upper = __builtin_amdgcn_readlane(v, 32);
+    const int lower = __builtin_amdgcn_readlane(v, 0);
+    return (threadIdx.x & 32) ? lower : upper;
+}
+
+template <int STEP> __device__ __forceinline__ int xlane(int v) {
+    if constexpr (STEP == 1)
+        return dpp_quad_xor1(v);
+    else if constexpr (STEP == 2)
+        return dpp_quad_xor2(v);
+    else if constexpr (STEP == 4)
+        return dpp_half_mirror(v);
+    else if constexpr (STEP == 8)
+        return dpp_mirror(v);
+    else if constexpr (STEP == 16)
+        return swz_xor16(v);
+    else
+        return perm_xor32(v);
+}
+
+template <int STEP> __device__ __forceinline__ float xlane_f(float v) {
+    return __int_as_float(xlane<STEP>(__float_as_int(v)));
+}
+template <int STEP> __device__ __forceinline__ double xlane_d(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = xlane<STEP>(int(bits & 0xffffffffll));
+    const int hi = xlane<STEP>(int(bits >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// Sum over the G-lane group in FP32 (plain form).
+template <int G> __device__ __forceinline__ float group_sum_f(float v) {
+    v += xlane_f<1>(v);
+    v += xlane_f<2>(v);
+    v += xlane_f<4>(v);
+    if constexpr (G >= 16)
+        v += xlane_f<8>(v);
+    if constexpr (G >= 32)
+        v += xlane_f<16>(v);
+    if constexpr (G >= 64)
+        v += xlane_f<32>(v);
+    return v;
+}
+// Sum over the G-lane group in FP64 (combination of FP32 compensated partials).
+template <int G> __device__ __forceinline__ double group_sum_d(double v) {
+    v += xlane_d<1>(v);
+    v += xlane_d<2>(v);
+    v += xlane_d<4>(v);
+    if constexpr (G >= 16)
+        v += xlane_d<8>(v);
+    if constexpr (G >= 32)
+        v += xlane_d<16>(v);
+    if constexpr (G >= 64)
+        v += xlane_d<32>(v);
+    return v;
+}
+// Bitwise OR / integer max / min over the group (validation flags, indices).
+template <int G> __device__ __forceinline__ int group_or(int v) {
+    v |= xlane<1>(v);
+    v |= xlane<2>(v);
+    v |= xlane<4>(v);
+    if constexpr (G >= 16)
+        v |= xlane<8>(v);
+    if constexpr (G >= 32)
+        v |= xlane<16>(v);
+    if constexpr (G >= 64)
+        v |= xlane<32>(v);
+    return v;
+}
+template <int G> __device__ __forceinline__ int group_max_i(int v) {
+    v = max(v, xlane<1>(v));
+    v = max(v, xlane<2>(v));
+    v = max(v, xlane<4>(v));
+    if constexpr (G >= 16)
+        v = max(v, xlane<8>(v));
+    if constexpr (G >= 32)
+        v = max(v, xlane<16>(v));
+    if constexpr (G >= 64)
+        v = max(v, xlane<32>(v));
+    return v;
+}
+template <int G> __device__ __forceinline__ int group_min_i(int v) {
+    v = min(v, xlane<1>(v));
+    v = min(v, xlane<2>(v));
+    v = min(v, xlane<4>(v));
+    if constexpr (G >= 16)
+        v = min(v, xlane<8>(v));
+    if constexpr (G >= 32)
+        v = min(v, xlane<16>(v));
+    if constexpr (G >= 64)
+        v = min(v, xlane<32>(v));
+    return v;
+}
+template <int G> __device__ __forceinline__ float group_min_f(float v) {
+    v = fminf(v, xlane_f<1>(v));
+    v = fminf(v, xlane_f<2>(v));
+    v = fminf(v, xlane_f<4>(v));
+    if constexpr (G >= 16)
+        v = fminf(v, xlane_f<8>(v));
+    if constexpr (G >= 32)
+        v = fminf(v, xlane_f<16>(v));
+    if constexpr (G >= 64)
+        v = fminf(v, xlane_f<32>(v));
+    return v;
+}
+
+}  // namespace bfhip

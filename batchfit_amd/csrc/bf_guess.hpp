@@ -1,0 +1,422 @@
+// bf_guess.hpp — data-only initial guess on the device, one 64-lane wave per
+// fit (cpu/initial_guess.hpp:120-249).
+//
+// The fit's signal is staged in LDS; edge medians come from bitonic sorts in
+// LDS; every scan of the reference (smoothed-maximum search, half-height
+// walks, prominence walks, valley search) becomes a wave-wide reduction with
+// the reference's tie rule (first index wins). The O(1) arithmetic between
+// scans runs redundantly on all lanes with FP contraction disabled, so the
+// FP32 result equals the FP32 reference guess bit for bit.
+#pragma once
+
+#include "bf_lm.hpp"
+
+namespace bfhip {
+
+struct GuessArgs {
+    const float *x;
+    const float *y;
+    long long ys_fit, ys_pt;
+    const GridInfo *grid;
+    uint32_t n, m;
+    uint32_t m_pad;   // LDS floats reserved for the signal
+    uint32_t sort_n;  // LDS floats reserved for sorting (power of two ≥ 2·count)
+    float *params;    // [N,P]
+    float *baseline;  // [N] or null
+    int32_t *gstatus; // [N] or null
+};
+
+enum : int32_t { GS_SUCCESS = 0, GS_INVALID = 1, GS_NO_PEAK = 2, GS_INSUFFICIENT = 3 };
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ float std_midpoint(float a, float b) {
+    const float small = 1.17549435082228750797e-38f * 2.0f;
+    const float big = 3.40282346638528859812e+38f / 2.0f;
+    const float aa = a < 0 ? -a : a, ab = b < 0 ? -b : b;
+    if (aa <= big && ab <= big)
+        return (a + b) / 2.0f;
+    if (aa < small)
+        return a + b / 2.0f;
+    if (ab < small)
+        return a / 2.0f + b;
+    return a / 2.0f + b / 2.0f;
+}
+__device__ __forceinline__ float std_lerp(float a, float b, float t) {
+    if ((a <= 0 && b >= 0) || (a >= 0 && b <= 0))
+        return t * b + (1 - t) * a;
+    if (t == 1)
+        return b;
+    const float v = a + t * (b - a);
+    return (t > 1) == (b > a) ? (b < v ? v : b) : (b > v ? v : b);
+}
+
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // one wave per block
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_or_i(int v) {
+    for (int o = 32; o > 0; o >>= 1)
+        v |= __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_min_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = (w < v) ? w : v;
+    }
+    return v;
+}
+// Argmax by value, ties to the smallest index; idx == INT_MAX means none.
+__device__ __forceinline__ void wave_argmax(float &v, int &idx) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        const int wi = __shfl_xor(idx, o, 64);
+        if (w > v || (w == v && wi < idx)) {
+            v = w;
+            idx = wi;
+        }
+    }
+}
+__device__ __forceinline__ void wave_argmin(float &v, int &idx) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        const int wi = __shfl_xor(idx, o, 64);
+        if (w < v || (w == v && wi < idx)) {
+            v = w;
+            idx = wi;
+        }
+    }
+}
+
+// Ascending bitonic sort of buf[0..n), n a power of two, padded by +inf.
+__device__ void wave_sort(float *buf, int n, int lane) {
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < n; i += 64) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const float a = buf[i], b = buf[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        buf[i] = b;
+                        buf[ixj] = a;
+                    }
+                }
+            }
+            wave_sync();
+        }
+}
+
+__device__ __forceinline__ int pow2_at_least(int v) {
+    int n = 1;
+    while (n < v)
+        n <<= 1;
+    return n;
+}
+
+// Median of src[0..count) (values provided by `get`), via a sort in buf
+// (initial_guess.hpp:25-30).
+template <typename F>
+__device__ float wave_median(float *buf, int count, int lane, F &&get) {
+    const int n = pow2_at_least(count);
+    for (int i = lane; i < n; i += 64)
+        buf[i] = i < count ? get(i) : __uint_as_float(0x7f800000u);
+    wave_sync();
+    wave_sort(buf, n, lane);
+    const float r = count % 2 ? buf[count / 2] : std_midpoint(buf[count / 2 - 1], buf[count / 2]);
+    wave_sync();
+    return r;
+}
+
+struct GuessPeak {
+    float amplitude, center, width, left_radius, right_radius;
+    bool valid;
+};
+
+__device__ __forceinline__ float smooth_at(const float *s, int m, int i) {
+    if (i == 0 || i + 1 == m)
+        return s[i];
+    return 0.25f * s[i - 1] + 0.5f * s[i] + 0.25f * s[i + 1];
+}
+
+// initial_guess.hpp:61-109 with both half-height walks as wave reductions.
+__device__ GuessPeak measure_peak_dev(const float *x, const float *s, int m, int index, int begin,
+                                      int end, bool mirror, int lane) {
+    GuessPeak r{0, 0, 0, 0, 0, false};
+    r.amplitude = s[index];
+    if (!(r.amplitude > 0.0f))
+        return r;
+    const float half = r.amplitude / 2.0f;
+    int left_c = -1, right_c = 0x7fffffff;
+    for (int j = begin + lane; j < index; j += 64)
+        if (s[j] <= half)
+            left_c = j;
+    for (int j = index + 1 + lane; j <= end; j += 64)
+        if (s[j] <= half && j < right_c)
+            right_c = j;
+    left_c = wave_max_i(left_c);
+    right_c = wave_min_i(right_c);
+    const int left = left_c >= 0 ? left_c : begin;
+    const int right = right_c != 0x7fffffff ? right_c : end;
+    const bool has_left = s[left] <= half, has_right = s[right] <= half;
+    if ((!has_left && (!mirror || begin == 0)) || (!has_right && (!mirror || end + 1 == m)) ||
+        (!has_left && !has_right))
+        return r;
+    int low = index > 2 ? index - 2 : 0;
+    int high = min(index + 2, end);
+    low = max(low, has_left ? left + 1 : begin);
+    high = min(high, has_right ? right - 1 : end);
+    float total = 0.0f;
+    r.center = x[index];
+    for (int j = low; j <= high; ++j) {
+        const float sj = s[j];
+        const float w = (0.0f < sj ? sj : 0.0f) / r.amplitude;
+        if (w > 0.0f) {
+            total += w;
+            r.center = std_lerp(r.center, x[j], w / total);
+        }
+    }
+    auto crossing = [&](int a, int b) {
+        const float den = s[b] - s[a];
+        if (!finite_f(den) || den == 0.0f)
+            return qnan_f();
+        return std_lerp(x[a], x[b], (half - s[a]) / den);
+    };
+    if (has_left)
+        r.left_radius = r.center - crossing(left, left + 1);
+    if (has_right)
+        r.right_radius = crossing(right - 1, right) - r.center;
+    if (!has_left)
+        r.left_radius = r.right_radius;
+    if (!has_right)
+        r.right_radius = r.left_radius;
+    r.width = r.left_radius + r.right_radius;
+    r.valid = finite_f(r.center) && finite_f(r.width) && r.left_radius > 0.0f &&
+              r.right_radius > 0.0f;
+    return r;
+}
+
+__device__ __forceinline__ int raw_peak_dev(const float *s, int i) {
+    int best = i - 1;
+    for (int j = i; j <= i + 1; ++j)
+        if (s[best] < s[j])
+            best = j;
+    return best;
+}
+
+// Prominence of the smoothed maximum at j (initial_guess.hpp:38-54).
+__device__ float prominence_dev(const float *s, int m, int j, int lane) {
+    const float height = smooth_at(s, m, j);
+    int kl = -1, kr = 0x7fffffff;
+    for (int k = lane; k < j; k += 64)
+        if (smooth_at(s, m, k) > height)
+            kl = k;
+    for (int k = j + 1 + lane; k < m; k += 64)
+        if (smooth_at(s, m, k) > height && k < kr)
+            kr = k;
+    kl = wave_max_i(kl);
+    kr = wave_min_i(kr);
+    float left = height, right = height;
+    for (int k = kl + 1 + lane; k < j; k += 64) {
+        const float v = smooth_at(s, m, k);
+        left = (v < left) ? v : left;
+    }
+    const int stop = kr == 0x7fffffff ? m : kr;
+    for (int k = j + 1 + lane; k < stop; k += 64) {
+        const float v = smooth_at(s, m, k);
+        right = (v < right) ? v : right;
+    }
+    left = wave_min_f(left);
+    right = wave_min_f(right);
+    return height - ((left < right) ? right : left);
+}
+
+template <int MODEL> __global__ void __launch_bounds__(64) guess_kernel(GuessArgs a) {
+    constexpr int P = Params<MODEL>::P;
+    constexpr bool paired = MODEL == TWO_LORENTZIAN;
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x;
+    const uint32_t fit = blockIdx.x;
+    if (fit >= a.n)
+        return;
+    const int m = int(a.m);
+    float *s = lds;
+    float *buf = lds + a.m_pad;
+    const float *x = a.x;
+    const float *y = a.y + fit * a.ys_fit;
+    const GridInfo grid = *a.grid;
+
+    float out[8];
+    for (int k = 0; k < 8; ++k)
+        out[k] = qnan_f();
+    float baseline = qnan_f();
+    int32_t gstatus = GS_INVALID;
+    auto finish = [&]() {
+        if (lane == 0) {
+            for (int k = 0; k < P; ++k)
+                a.params[size_t(fit) * P + k] = out[k];
+            if (a.baseline)
+                a.baseline[fit] = baseline;
+            if (a.gstatus)
+                a.gstatus[fit] = gstatus;
+        }
+    };
+    if (m < 5 || !grid.guess_grid_ok)
+        return finish();
+    int bad = 0;
+    for (int j = lane; j < m; j += 64) {
+        const float v = y[j * a.ys_pt];
+        bad |= !finite_f(v);
+        s[j] = v;
+    }
+    bad = wave_or_i(bad);
+    wave_sync();
+    if (bad)
+        return finish();
+    const int count = grid.count;
+    const float left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
+    const float right_level = wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
+    baseline = wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? s[i] : s[m - count + (i - count)];
+    });
+    const float left_x = grid.left_x, right_x = grid.right_x;
+    const float slope = paired ? (right_level - left_level) / (right_x - left_x) : 0.0f;
+    const float level = paired ? left_level : baseline;
+    if (!finite_f(slope))
+        return finish();
+    const float noise = 1.4826f * wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? fabsf(s[i] - left_level) : fabsf(s[m - count + (i - count)] - right_level);
+    });
+    bad = 0;
+    for (int j = lane; j < m; j += 64) {
+        const float v = s[j] - (level + slope * (x[j] - left_x));
+        bad |= !finite_f(v);
+        s[j] = v;
+    }
+    bad = wave_or_i(bad);
+    wave_sync();
+    if (bad || !finite_f(noise))
+        return finish();
+
+    // Strongest smoothed local maximum (initial_guess.hpp:170-182).
+    float best = 0.0f;
+    int first = 0x7fffffff;
+    for (int j = 1 + lane; j + 1 < m; j += 64) {
+        const float v = smooth_at(s, m, j);
+        if (v > best && v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1)) {
+            best = v;
+            first = j;
+        }
+    }
+    wave_argmax(best, first);
+    gstatus = GS_NO_PEAK;
+    if (first == 0x7fffffff || best <= 3.0f * noise)
+        return finish();
+    first = raw_peak_dev(s, first);
+    GuessPeak primary = measure_peak_dev(x, s, m, first, 0, m - 1, false, lane);
+    if (!primary.valid)
+        return finish();
+    if constexpr (!paired) {
+        out[0] = primary.amplitude;
+        out[1] = primary.center;
+        if constexpr (MODEL == GAUSSIAN)
+            out[2] = primary.width / 2.354820045030949382f;
+        else if constexpr (MODEL == LORENTZIAN)
+            out[2] = primary.width / 2.0f;
+        else {
+            out[2] = primary.width;
+            out[3] = 0.5f;
+        }
+    } else {
+        gstatus = GS_INSUFFICIENT;
+        const float lr = primary.left_radius, rr = primary.right_radius;
+        const float a3 = 3.0f * grid.guess_spacing, a2 = 2.0f * ((rr < lr) ? rr : lr);
+        const float separation = (a3 < a2) ? a2 : a3;
+        // Candidates in decreasing (value, −index) order until one is
+        // prominent enough (initial_guess.hpp:203-217).
+        float prev_v = __uint_as_float(0x7f800000u);
+        int prev_j = -1;
+        int second = -1;
+        for (;;) {
+            float v_best = 0.0f;
+            int j_best = 0x7fffffff;
+            for (int j = 1 + lane; j + 1 < m; j += 64) {
+                const float v = smooth_at(s, m, j);
+                if (!(v > 0.0f) || v <= smooth_at(s, m, j - 1) || v < smooth_at(s, m, j + 1))
+                    continue;
+                if (!(v < prev_v || (v == prev_v && j > prev_j)))
+                    continue;
+                const int idx = raw_peak_dev(s, j);
+                const int bins = idx > first ? idx - first : first - idx;
+                if (bins < 3 || fabsf(x[idx] - primary.center) < separation)
+                    continue;
+                if (v > v_best || (v == v_best && j < j_best)) {
+                    v_best = v;
+                    j_best = j;
+                }
+            }
+            wave_argmax(v_best, j_best);
+            if (j_best == 0x7fffffff)
+                break;
+            if (prominence_dev(s, m, j_best, lane) > 3.0f * noise) {
+                second = raw_peak_dev(s, j_best);
+                break;
+            }
+            prev_v = v_best;
+            prev_j = j_best;
+        }
+        if (second < 0)
+            return finish();
+        if (first > second) {
+            const int t = first;
+            first = second;
+            second = t;
+        }
+        float vmin = __uint_as_float(0x7f800000u);
+        int valley = 0x7fffffff;
+        for (int j = first + 1 + lane; j < second; j += 64)
+            if (s[j] < vmin || (s[j] == vmin && j < valley)) {
+                vmin = s[j];
+                valley = j;
+            }
+        wave_argmin(vmin, valley);
+        if (valley == 0x7fffffff)
+            valley = second;
+        primary = measure_peak_dev(x, s, m, first, 0, valley, true, lane);
+        const GuessPeak secondary = measure_peak_dev(x, s, m, second, valley, m - 1, true, lane);
+        if (!primary.valid || !secondary.valid || !(primary.center < secondary.center))
+            return finish();
+        out[0] = primary.amplitude;
+        out[1] = primary.center;
+        out[2] = primary.width / 2.0f;
+        out[3] = secondary.amplitude;
+        out[4] = secondary.center;
+        out[5] = secondary.width / 2.0f;
+        out[6] = left_level - slope * left_x;
+        out[7] = slope;
+    }
+    for (int k = 0; k < P; ++k)
+        if (!finite_f(out[k])) {
+            gstatus = GS_INVALID;
+            return finish();
+        }
+    if (!(out[2] > 0.0f) || (paired && !(out[5] > 0.0f))) {
+        gstatus = GS_INVALID;
+        return finish();
+    }
+    gstatus = GS_SUCCESS;
+    finish();
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace bfhip

@@ -1,0 +1,632 @@
+// bf_lm.hpp — bounded Levenberg–Marquardt kernels for batches of independent
+// spectra on CDNA4 (gfx950).
+//
+// Mapping: a group of G lanes owns one fit (G = 1 lane-per-fit, or 8/16/32/64
+// lanes). With R > 0 the group holds its fit's points in registers, lane l of
+// the group owning points l, l+G, …, l+G(R-1); y is read from HBM once per fit.
+// With R = 0 every pass streams x and y from memory (lane-per-fit form).
+//
+// Algorithm: the reference's bounded LM (cpu/solver.hpp:196-342; GPU forms
+// src/metal/lm.metal:108-272 and cooperative.metal:232-388) with one change of
+// schedule that does not change any result in exact arithmetic: each LM
+// iteration makes ONE pass over the data at the trial point, producing χ², JᵀJ
+// and Jᵀr together. If the step is accepted, that linearisation is the next
+// iteration's; if it is rejected, the previous one is kept. The reference
+// instead recomputes the linearisation at the start of every iteration and
+// evaluates the trial cost in a separate pass.
+//
+// Numerics (bf_config.numerics):
+//   bit0 (1) compensated per-lane partials (Kahan, numerics.hpp:12-18)
+//   bit1 (2) equilibrated Cholesky (cpu/solver.hpp:101-113)
+//   bit2 (4) mixed precision: lane partials are combined across the group in
+//            FP64, and the P×P solve, accept/reject test, λ schedule and
+//            covariance inverse run in FP64. Without bit2 everything is FP32,
+//            as in the reference GPU kernels.
+// Model value, residual and Jacobian are always FP32.
+#pragma once
+
+#include "bf_device.hpp"
+
+namespace bfhip {
+
+// Grid facts shared by all fits, derived on the device from x
+// (cpu/solver.hpp:58-69 grid_spacing; batch.hpp:236-240 varied_x;
+//  initial_guess.hpp:131-153 guess preconditions and edge abscissae).
+struct GridInfo {
+    float min_spacing;      // grid_spacing(): min |Δx| if x is monotone, else 0
+    int32_t varied_x;       // any x[j] != x[0]
+    int32_t x_finite;       // all x finite
+    int32_t guess_grid_ok;  // strictly increasing, finite, finite span
+    float guess_spacing;    // min (x[j] - x[j-1])
+    float left_x, right_x;  // midpoints of the two edge windows
+    int32_t count;          // edge window length max(1, M/10)
+};
+
+struct FitArgs {
+    const float *x;
+    const float *y;
+    long long ys_fit, ys_pt;  // y[fit*ys_fit + j*ys_pt]
+    const float *p0;          // [N,P] initial parameters
+    float *params;
+    int32_t *status;
+    uint32_t *iterations;
+    float *chi2;
+    float *cov;  // [N,P,P] or null
+    float *stderr_;
+    int32_t *ustatus;
+    const GridInfo *grid;
+    uint32_t n, m;
+    uint32_t max_iterations;
+    float initial_lambda, gradient_tolerance, step_tolerance, cost_tolerance;
+    float lower[8], upper[8];
+    uint32_t numerics;
+    int32_t options_ok;  // λ and tolerances valid, bounds ordered/non-NaN/non-empty-side
+    int32_t free_shape;  // cpu/solver.hpp:70-82
+    float width_factor;  // cpu/solver.hpp:86-88
+};
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+    return (v < lo) ? lo : (hi < v) ? hi : v;  // std::clamp
+}
+template <typename S> __device__ __forceinline__ S smax(S a, S b) { return (a < b) ? b : a; }
+template <typename S> __device__ __forceinline__ S smin(S a, S b) { return (b < a) ? b : a; }
+template <typename S> __device__ __forceinline__ bool finite_s(S v) {
+    if constexpr (sizeof(S) == 8)
+        return finite_d(v);
+    else
+        return finite_f(v);
+}
+template <typename S> __device__ __forceinline__ S sabs(S v) { return v < S(0) ? -v : v; }
+
+// Packed lower-triangle index of (k, l), l <= k.
+__host__ __device__ constexpr int tri(int k, int l) { return k * (k + 1) / 2 + l; }
+
+// Checked, unpivoted Cholesky solve on a packed lower triangle
+// (cpu/solver.hpp:99-143). Returns false on a non-positive pivot, a
+// non-finite entry or a non-finite solution.
+template <typename S, int P>
+__device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
+    S d[P];
+    if (scaled) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            if (!finite_s(a[tri(i, i)]) || a[tri(i, i)] <= S(0))
+                return false;
+            d[i] = sqrt(a[tri(i, i)]);
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            b[i] /= d[i];
+#pragma unroll
+            for (int j = 0; j <= i; ++j)
+                a[tri(i, j)] = (a[tri(i, j)] / d[i]) / d[j];
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            S v = a[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                v -= a[tri(i, k)] * a[tri(j, k)];
+            ok = ok && finite_s(v) && !(i == j && v <= S(0));
+            a[tri(i, j)] = i == j ? sqrt(v) : v / a[tri(j, j)];
+        }
+        S v = b[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j)
+            v -= a[tri(i, j)] * b[j];
+        b[i] = v / a[tri(i, i)];
+    }
+#pragma unroll
+    for (int i = P - 1; i >= 0; --i) {
+        S v = b[i];
+#pragma unroll
+        for (int j = i + 1; j < P; ++j)
+            v -= a[tri(j, i)] * b[j];
+        b[i] = v / a[tri(i, i)];
+        ok = ok && finite_s(b[i]);
+    }
+    if (scaled)
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            b[i] /= d[i];
+            ok = ok && finite_s(b[i]);
+        }
+    return ok;
+}
+
+// Per-lane partial accumulation: FMA (plain) or Kahan (compensated).
+template <bool COMP> struct Acc;
+template <> struct Acc<false> {
+    float s;
+    __device__ __forceinline__ void zero() { s = 0.0f; }
+    __device__ __forceinline__ void add_prod(float a, float b) { s = __builtin_fmaf(a, b, s); }
+    __device__ __forceinline__ float hi() const { return s; }
+    __device__ __forceinline__ float lo() const { return 0.0f; }
+};
+template <> struct Acc<true> {
+    float s, c;  // true sum ≈ s − c
+    __device__ __forceinline__ void zero() { s = c = 0.0f; }
+    __device__ __forceinline__ void add_prod(float a, float b) {
+#pragma clang fp contract(off)
+        const float adjusted = a * b - c;
+        const float next = s + adjusted;
+        c = (next - s) - adjusted;
+        s = next;
+    }
+    __device__ __forceinline__ float hi() const { return s; }
+    __device__ __forceinline__ float lo() const { return c; }
+};
+
+// Group total of one term: FP64 in mixed mode, FP32 otherwise.
+template <int G, typename S, bool COMP>
+__device__ __forceinline__ S reduce_term(const Acc<COMP> &a) {
+    if constexpr (sizeof(S) == 8) {
+        double v = double(a.hi()) - double(a.lo());
+        if constexpr (G > 1)
+            v = group_sum_d<G>(v);
+        return v;
+    } else {
+        float v = a.hi();
+        if constexpr (G > 1)
+            v = group_sum_f<G>(v);
+        if constexpr (COMP) {
+            float c = a.lo();
+            if constexpr (G > 1)
+                c = group_sum_f<G>(c);
+            v -= c;
+        }
+        return v;
+    }
+}
+
+// Points of one fit as seen by one lane: a register tile (R > 0) or a
+// strided stream from memory (R == 0).
+template <int G, int R> struct Points {
+    float xv[R > 0 ? R : 1], yv[R > 0 ? R : 1];
+    const float *x, *y;
+    long long ys_pt;
+    int gl, m;
+    __device__ __forceinline__ void load(const float *x_, const float *yrow, long long ys_pt_,
+                                         int gl_, int m_) {
+        x = x_;
+        y = yrow;
+        ys_pt = ys_pt_;
+        gl = gl_;
+        m = m_;
+        if constexpr (R > 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = gl + G * r;
+                const bool in = j < m;
+                xv[r] = in ? x[j] : 0.0f;
+                yv[r] = in ? y[j * ys_pt] : 0.0f;
+            }
+        }
+    }
+    // f(j, x, y) for every point this lane owns.
+    template <typename F> __device__ __forceinline__ void each(F &&f) const {
+        if constexpr (R > 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (gl + G * r < m)
+                    f(xv[r], yv[r]);
+        } else {
+            for (int j = gl; j < m; j += G)
+                f(x[j], y[j * ys_pt]);
+        }
+    }
+};
+
+// One pass at parameters p: χ² (terms[TRI+P]), JᵀJ packed (terms[0..TRI)),
+// Jᵀr (terms[TRI..TRI+P)), reduced over the group. `lin_ok` is the finiteness
+// check of the linearisation (cpu/solver.hpp:182-190).
+template <int MODEL, int G, int R, bool COMP, typename S>
+__device__ __forceinline__ void fused_pass(const Points<G, R> &pts, const float *p, S *terms,
+                                           bool &lin_ok) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI, T = Params<MODEL>::TERMS;
+    ModelConsts<MODEL> mc;
+    mc.set(p);
+    Acc<COMP> acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+        acc[t].zero();
+    pts.each([&](float xj, float yj) {
+        float jac[P];
+        const float f = mc.eval(xj, jac);
+        const float r = yj - f;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+#pragma unroll
+            for (int l = 0; l <= k; ++l)
+                acc[tri(k, l)].add_prod(jac[k], jac[l]);
+            acc[TRI + k].add_prod(jac[k], r);
+        }
+        acc[TRI + P].add_prod(r, r);
+    });
+    lin_ok = true;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        terms[t] = reduce_term<G, S, COMP>(acc[t]);
+        if (t < TRI + P)
+            lin_ok = lin_ok && finite_s(terms[t]);
+    }
+}
+
+// JᵀJ only (uncertainty at the accepted point), always compensated per lane
+// (cpu/uncertainty.hpp:297-309); `jac_ok` is false if any Jacobian entry is
+// non-finite.
+template <int MODEL, int G, int R, typename S>
+__device__ __forceinline__ void information_pass(const Points<G, R> &pts, const float *p,
+                                                 S *info, bool &jac_ok) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    ModelConsts<MODEL> mc;
+    mc.set(p);
+    Acc<true> acc[TRI];
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        acc[t].zero();
+    int bad = 0;
+    pts.each([&](float xj, float) {
+        float jac[P];
+        (void)mc.eval(xj, jac);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            bad |= !finite_f(jac[k]);
+#pragma unroll
+            for (int l = 0; l <= k; ++l)
+                acc[tri(k, l)].add_prod(jac[k], jac[l]);
+        }
+    });
+    if constexpr (G > 1)
+        bad = group_or<G>(bad);
+    jac_ok = !bad;
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        info[t] = reduce_term<G, S, true>(acc[t]);
+}
+
+template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const float *p,
+                                                                        float spacing,
+                                                                        float factor) {
+    if (spacing > 0.0f && fabsf(p[2]) * factor < spacing)
+        return ST_UNRESOLVED;
+    if constexpr (Params<MODEL>::P == 8)
+        if (spacing > 0.0f && fabsf(p[5]) * factor < spacing)
+            return ST_UNRESOLVED;
+    return ST_CONVERGED;
+}
+
+// Covariance at the accepted point (cpu/uncertainty.hpp:13-110), written for
+// one fit by lane 0 of its group.
+template <int MODEL, int G, int R, typename S>
+__device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R> &pts, uint32_t fit,
+                                           int gl, const float *p, int32_t status, float chi2) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    const float nan = qnan_f();
+    float *cov = a.cov + size_t(fit) * P * P;
+    float *se = a.stderr_ + size_t(fit) * P;
+    auto fail = [&](int32_t us) {
+        if (gl == 0) {
+            for (int k = 0; k < P * P; ++k)
+                cov[k] = nan;
+            for (int k = 0; k < P; ++k)
+                se[k] = nan;
+            a.ustatus[fit] = us;
+        }
+    };
+    if (status != ST_CONVERGED)
+        return fail(US_NOT_CONVERGED);
+    if (!finite_f(chi2) || chi2 < 0.0f)
+        return fail(US_INVALID);
+    bool fixed[P];
+    int free = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        if (!finite_f(p[k]) || nan_f(a.lower[k]) || nan_f(a.upper[k]) || a.lower[k] > a.upper[k] ||
+            p[k] < a.lower[k] || p[k] > a.upper[k])
+            return fail(US_INVALID);
+        fixed[k] = a.lower[k] == a.upper[k];
+        free += !fixed[k];
+        if (!fixed[k] && (p[k] == a.lower[k] || p[k] == a.upper[k]))
+            return fail(US_ACTIVE_BOUND);
+    }
+    if (int(a.m) <= free)
+        return fail(US_DOF);
+    if (!a.grid->x_finite)
+        return fail(US_INVALID);
+    S info[TRI];
+    bool jac_ok;
+    information_pass<MODEL, G, R, S>(pts, p, info, jac_ok);
+    if (!jac_ok)
+        return fail(US_INVALID);
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        if (!finite_s(info[t]))
+            return fail(US_INVALID);
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+#pragma unroll
+        for (int l = 0; l <= i; ++l)
+            if (fixed[i] || fixed[l])
+                info[tri(i, l)] = i == l ? S(1) : S(0);
+    // The variance uses the reported FP32 χ², as the FP32 reference does.
+    const S variance = S(chi2) / S(float(a.m - free));
+    if (!finite_s(variance))
+        return fail(US_INVALID);
+    // The factor does not depend on the right-hand side: factor once, then
+    // solve each unit column (the reference refactors per column; same values).
+    S factor[TRI];
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        factor[t] = info[t];
+    S unit[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        unit[k] = S(0);
+    // Factor with a zero right-hand side to obtain the scaled factor.
+    if (!chol_solve<S, P>(factor, unit, true))
+        return fail(US_SINGULAR);
+    S d[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        d[i] = sqrt(info[tri(i, i)]);
+        if (!fixed[i] && factor[tri(i, i)] * factor[tri(i, i)] <= S(64) * S(1.1920928955078125e-7))
+            return fail(US_SINGULAR);
+    }
+    S c[P * P];
+#pragma unroll
+    for (int col = 0; col < P; ++col) {
+        S b[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+            b[i] = (i == col ? S(1) : S(0)) / d[i];
+        // forward and back substitution with the equilibrated factor
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            S v = b[i];
+#pragma unroll
+            for (int j = 0; j < i; ++j)
+                v -= factor[tri(i, j)] * b[j];
+            b[i] = v / factor[tri(i, i)];
+        }
+        bool ok = true;
+#pragma unroll
+        for (int i = P - 1; i >= 0; --i) {
+            S v = b[i];
+#pragma unroll
+            for (int j = i + 1; j < P; ++j)
+                v -= factor[tri(j, i)] * b[j];
+            b[i] = v / factor[tri(i, i)];
+            ok = ok && finite_s(b[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            b[i] /= d[i];
+            ok = ok && finite_s(b[i]);
+        }
+        if (fixed[col]) {
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                c[i * P + col] = S(0);
+            continue;
+        }
+        if (!ok)
+            return fail(US_SINGULAR);
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            c[i * P + col] = fixed[i] ? S(0) : b[i] * variance;
+            if (!finite_s(c[i * P + col]))
+                return fail(US_INVALID);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        if (c[i * P + i] < S(0))
+            return fail(US_INVALID);
+    if (gl == 0) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const S sym = i == j ? c[i * P + i] : c[i * P + j] / S(2) + c[j * P + i] / S(2);
+                cov[i * P + j] = float(sym);
+            }
+            se[i] = float(sqrt(c[i * P + i]));
+        }
+        a.ustatus[fit] = US_AVAILABLE;
+    }
+}
+
+// The LM driver for one fit, run redundantly by every lane of its group.
+template <int MODEL, int G, int R, bool COMP, typename S>
+__device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    const GridInfo grid = *a.grid;
+    Points<G, R> pts;
+    pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
+
+    float init[P], p[P];
+    bool bad = !a.options_ok || !grid.x_finite;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        init[k] = a.p0[size_t(fit) * P + k];
+        bad = bad || !finite_f(init[k]);
+    }
+    // Per-fit observation checks: finiteness and flatness (lm.metal:133-139).
+    const float y0 = a.y[fit * a.ys_fit];
+    int flags = 0;
+    pts.each([&](float, float yj) {
+        flags |= finite_f(yj) ? 0 : 1;
+        flags |= (yj != y0) ? 2 : 0;
+    });
+    if constexpr (G > 1)
+        flags = group_or<G>(flags);
+    bad = bad || (flags & 1);
+
+    float *out_p = a.params + size_t(fit) * P;
+    auto write = [&](const float *q, int32_t st, uint32_t it, float c2) {
+        if (gl == 0) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                out_p[k] = q[k];
+            a.status[fit] = st;
+            a.iterations[fit] = it;
+            a.chi2[fit] = c2;
+        }
+    };
+    if (bad) {
+        write(init, ST_INVALID, 0, qnan_f());
+        if (a.cov)
+            covariance<MODEL, G, R, S>(a, pts, fit, gl, init, ST_INVALID, qnan_f());
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        p[k] = clampf(init[k], a.lower[k], a.upper[k]);
+
+    S terms[Params<MODEL>::TERMS];
+    bool lin_ok;
+    fused_pass<MODEL, G, R, COMP, S>(pts, p, terms, lin_ok);
+    S chi = terms[TRI + P];
+    int32_t status = ST_MAX_ITERATIONS;
+    uint32_t iterations = 0;
+    if (!finite_s(chi)) {
+        write(p, ST_INVALID, 0, qnan_f());
+        if (a.cov)
+            covariance<MODEL, G, R, S>(a, pts, fit, gl, p, ST_INVALID, qnan_f());
+        return;
+    }
+    if (!(flags & 2) && grid.varied_x && a.free_shape) {
+        write(p, ST_DEGENERATE, 0, float(chi));
+        if (a.cov)
+            covariance<MODEL, G, R, S>(a, pts, fit, gl, p, ST_DEGENERATE, float(chi));
+        return;
+    }
+    S A[TRI], g[P];
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        A[t] = terms[t];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        g[k] = terms[TRI + k];
+
+    constexpr float eps = 1.1920928955078125e-7f;  // FLT_EPSILON: the options are FP32
+    const S lambda0 = S(a.initial_lambda);
+    const S lambda_cap = smax(lambda0, S(1.0f / eps));
+    const S gtol = S(a.gradient_tolerance), stol = S(a.step_tolerance),
+            ctol = S(a.cost_tolerance);
+    const bool scaled = (a.numerics & 2u) != 0;
+    S lambda = lambda0;
+    for (uint32_t it = 0; it < a.max_iterations; ++it) {
+        if (!lin_ok) {
+            status = ST_INVALID;
+            break;
+        }
+        bool pinned[P];
+        S gnorm = S(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            pinned[k] = a.lower[k] == a.upper[k] || (p[k] == a.lower[k] && g[k] < S(0)) ||
+                        (p[k] == a.upper[k] && g[k] > S(0));
+            if (!pinned[k])
+                gnorm = smax(gnorm, sabs(g[k]));
+        }
+        if (gnorm <= gtol) {
+            status = resolved_status<MODEL>(p, grid.min_spacing, a.width_factor);
+            break;
+        }
+        S damped[TRI], step[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            step[k] = pinned[k] ? S(0) : g[k];
+#pragma unroll
+            for (int l = 0; l <= k; ++l)
+                damped[tri(k, l)] = (pinned[k] || pinned[l]) ? (k == l ? S(1) : S(0)) : A[tri(k, l)];
+            if (!pinned[k])
+                damped[tri(k, k)] += lambda * A[tri(k, k)];
+        }
+        iterations = it + 1;
+        if (!chol_solve<S, P>(damped, step, scaled)) {
+            status = ST_SINGULAR;
+            break;
+        }
+        float trial[P];
+        S step_norm = S(0), param_norm = S(0);
+        bool finite_trial = true;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            step_norm = smax(step_norm, sabs(step[k]));
+            trial[k] = clampf(float(S(p[k]) + step[k]), a.lower[k], a.upper[k]);
+            finite_trial = finite_trial && finite_f(trial[k]);
+            step[k] = S(trial[k]) - S(p[k]);
+            param_norm = smax(param_norm, S(fabsf(p[k])));
+        }
+        if (step_norm <= stol * (param_norm + stol) / smax(S(1), lambda)) {
+            status = resolved_status<MODEL>(p, grid.min_spacing, a.width_factor);
+            break;
+        }
+        S predicted = S(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            predicted += S(2) * g[k] * step[k];
+#pragma unroll
+            for (int l = 0; l < P; ++l)
+                predicted -= step[k] * A[k >= l ? tri(k, l) : tri(l, k)] * step[l];
+        }
+        S trial_chi = S(qnan_f());
+        bool trial_ok = false;
+        if (finite_trial) {
+            fused_pass<MODEL, G, R, COMP, S>(pts, trial, terms, trial_ok);
+            trial_chi = terms[TRI + P];
+        }
+        const S reduction = chi - trial_chi;
+        const S threshold = ctol * chi;
+        const bool small_change = finite_s(trial_chi) && finite_s(predicted) &&
+                                  predicted >= S(0) && sabs(reduction) <= threshold &&
+                                  predicted <= threshold / smax(S(1), lambda);
+        if (finite_s(trial_chi) && reduction > S(0) && finite_s(predicted) && predicted > S(0)) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                p[k] = trial[k];
+            chi = trial_chi;
+#pragma unroll
+            for (int t = 0; t < TRI; ++t)
+                A[t] = terms[t];
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                g[k] = terms[TRI + k];
+            lin_ok = trial_ok;
+            const S rho = reduction / predicted;
+            const S q = S(2) * rho - S(1);
+            const S factor = rho >= S(1) ? S(1) / S(3) : smax(S(1) / S(3), S(1) - q * q * q);
+            const S next = lambda * factor;
+            lambda = next < S(eps) ? S(eps) : (lambda_cap < next ? lambda_cap : next);
+        } else {
+            lambda = smin(lambda_cap, lambda * S(10));
+        }
+        if (small_change) {
+            status = resolved_status<MODEL>(p, grid.min_spacing, a.width_factor);
+            break;
+        }
+    }
+    const float chi_out = float(chi);
+    write(p, status, iterations, chi_out);
+    if (a.cov)
+        covariance<MODEL, G, R, S>(a, pts, fit, gl, p, status, chi_out);
+}
+
+template <int MODEL, int G, int R, bool COMP, bool MIXED>
+__global__ void __launch_bounds__(256) lm_kernel(FitArgs a) {
+    constexpr int per_block = 256 / G;
+    const uint32_t fit = blockIdx.x * per_block + threadIdx.x / G;
+    if (fit >= a.n)
+        return;
+    using S = typename std::conditional<MIXED, double, float>::type;
+    solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x % G));
+}
+
+}  // namespace bfhip

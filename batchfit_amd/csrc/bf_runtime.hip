@@ -1,0 +1,637 @@
+// bf_runtime.hip — the C ABI of libbatchfit_hip.so (include/batchfit_hip.h):
+// argument checks, stream-ordered scratch, the grid prologue, the data-only
+// guess, the optional point-major transpose and the LM launch.
+//
+// Replaces the reference's Metal host runtime (src/metal/runtime.cpp:26-316)
+// and the binding's host preparation (python/bindings.cpp:60-121): the guess
+// runs on the device instead of a 2-worker CPU pool, and y stays fit-major in
+// HBM for the group mappings instead of a host 16×16 transpose.
+#include "bf_launch.hpp"
+
+#include "../../include/batchfit_hip.h"
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+
+namespace bfhip {
+
+template <int MODEL> hipError_t launch_lm_model(const Plan &, const FitArgs &, hipStream_t);
+template <int MODEL> hipError_t launch_guess_model(const GuessArgs &, size_t, hipStream_t);
+template <int MODEL> hipError_t launch_eval_model(const EvalArgs &, hipStream_t);
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device kernels owned by the runtime.
+
+// Shared-grid facts (cpu/solver.hpp:58-69, batch.hpp:236-240,
+// initial_guess.hpp:131-153) in one 256-thread block. Minima are exact, so the
+// parallel order gives the reference's sequential values.
+__global__ void __launch_bounds__(256) grid_kernel(const float *x, uint32_t m, GridInfo *out) {
+    __shared__ float s_abs[256], s_gap[256];
+    __shared__ int s_flags[256];
+    const int t = threadIdx.x;
+    const float x0 = x[0];
+    float amin = __uint_as_float(0x7f800000u), gmin = 3.40282346638528859812e+38f;
+    int fl = 0;  // 1 non-finite, 2 not rising, 4 not falling, 8 varied, 16 not strictly rising
+    for (uint32_t j = t; j < m; j += 256) {
+        const float xj = x[j];
+        fl |= finite_f(xj) ? 0 : 1;
+        fl |= (xj != x0) ? 8 : 0;
+        if (j > 0) {
+            const float prev = x[j - 1];
+            const float d = xj - prev;
+            fl |= (d >= 0.0f) ? 0 : 2;
+            fl |= (d <= 0.0f) ? 0 : 4;
+            if (d != 0.0f)
+                amin = fminf(amin, fabsf(d));
+            fl |= (xj <= prev) ? 16 : 0;
+            gmin = fminf(gmin, d);
+        }
+    }
+    s_abs[t] = amin;
+    s_gap[t] = gmin;
+    s_flags[t] = fl;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            s_abs[t] = fminf(s_abs[t], s_abs[t + o]);
+            s_gap[t] = fminf(s_gap[t], s_gap[t + o]);
+            s_flags[t] |= s_flags[t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int f = s_flags[0];
+        const bool monotone = !(f & 2) || !(f & 4);
+        GridInfo g;
+        g.min_spacing = monotone && finite_f(s_abs[0]) ? s_abs[0] : 0.0f;
+        g.varied_x = (f & 8) ? 1 : 0;
+        g.x_finite = (f & 1) ? 0 : 1;
+        g.guess_grid_ok = !(f & 1) && !(f & 16) && finite_f(x[m - 1] - x[0]);
+        g.guess_spacing = s_gap[0];
+        const uint32_t count = m / 10 > 1 ? m / 10 : 1;
+        g.count = int(count);
+        g.left_x = std_midpoint(x[(count - 1) / 2], x[count / 2]);
+        g.right_x = std_midpoint(x[m - count + (count - 1) / 2], x[m - count + count / 2]);
+        *out = g;
+    }
+}
+
+// Fit-major [n,m] → point-major [m,n] (the layout of the reference GPU
+// kernels, batch.hpp:119), 64×64 tiles through LDS.
+__global__ void __launch_bounds__(256) transpose_kernel(const float *in, float *out, uint32_t n,
+                                                        uint32_t m) {
+    __shared__ float tile[64][65];
+    const uint32_t fit0 = blockIdx.x * 64, pt0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int k = ty; k < 64; k += 4) {
+        const uint32_t i = fit0 + k, j = pt0 + tx;
+        if (i < n && j < m)
+            tile[k][tx] = in[size_t(i) * m + j];
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const uint32_t j = pt0 + k, i = fit0 + tx;
+        if (i < n && j < m)
+            out[size_t(j) * n + i] = tile[tx][k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+constexpr int kParams[4] = {3, 3, 4, 8};
+
+int fail(char *err, size_t errlen, int code, const std::string &message) {
+    if (err && errlen) {
+        std::snprintf(err, errlen, "%s", message.c_str());
+    }
+    return code;
+}
+
+#define BF_TRY(expr, what)                                                                     \
+    do {                                                                                       \
+        const hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(err, errlen, BF_RUNTIME_ERROR,                                         \
+                        std::string(what) + " failed: " + hipGetErrorString(e_));             \
+    } while (0)
+
+struct Runtime {
+    int status = BF_UNAVAILABLE;
+    std::string why = "no HIP device";
+    hipStream_t stream = nullptr;
+    std::mutex host_calls;  // serialises the synchronous host-array entry (bindings.cpp:13)
+
+    Runtime() {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            why = "no HIP device is visible";
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            why = "hipGetDeviceProperties failed";
+            return;
+        }
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            why = std::string("device 0 is ") + prop.gcnArchName + ", this library targets gfx950";
+            return;
+        }
+        if (hipSetDevice(0) != hipSuccess ||
+            hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            why = "could not create a HIP stream";
+            return;
+        }
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
+            uint64_t keep = std::numeric_limits<uint64_t>::max();
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        status = BF_OK;
+        why.clear();
+    }
+    static Runtime &get() {
+        static Runtime runtime;
+        return runtime;
+    }
+};
+
+std::atomic<bool> g_timing{std::getenv("BF_HIP_TIMING") != nullptr};
+
+struct Timing {
+    hipEvent_t ev[5] = {};
+    bool recorded = false;
+    bool phase[4] = {};
+};
+thread_local Timing t_timing;
+
+hipError_t record(int i, hipStream_t stream) {
+    if (!t_timing.ev[i]) {
+        const hipError_t e = hipEventCreate(&t_timing.ev[i]);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipEventRecord(t_timing.ev[i], stream);
+}
+
+// Tuned default mapping by (model, M) (replaces metal::dispatch_variant,
+// include/batchfit/metal/batch.hpp:21-50). Every row keeps y in registers.
+int pow2_ceil(uint32_t v) {
+    int r = 1;
+    while (uint32_t(r) < v)
+        r <<= 1;
+    return r;
+}
+
+int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *launch, Plan &plan,
+                char *err, size_t errlen) {
+    plan.comp = (numerics & BF_COMPENSATED) != 0;
+    plan.mixed = (numerics & BF_MIXED_FP64) != 0;
+    const uint32_t mapping = launch ? launch->mapping : BF_MAP_AUTO;
+    if (mapping == BF_MAP_LANE) {
+        plan.G = 1;
+        plan.R = 0;
+        return BF_OK;
+    }
+    int g = 64;
+    if (mapping == BF_MAP_GROUP && launch->group_lanes)
+        g = int(launch->group_lanes);
+    else if (mapping != BF_MAP_GROUP && mapping != BF_MAP_AUTO)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown mapping");
+    (void)model;
+    if (g != 16 && g != 32 && g != 64)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "group_lanes must be 16, 32 or 64");
+    const uint32_t per_lane = (m + uint32_t(g) - 1) / uint32_t(g);
+    int r = pow2_ceil(per_lane);
+    if (r > kMaxTile) {
+        if (g != 64)
+            return fail(err, errlen, BF_INVALID_ARGUMENT,
+                        "M exceeds the register tile of this group size; use 64 lanes");
+        r = 0;
+    }
+    plan.G = g;
+    plan.R = r;
+    if (!plan_instantiated(plan.G, plan.R, plan.comp, plan.mixed))
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "this group size supports numerics 0-3 (FP32) and 5/7 (compensated mixed)");
+    return BF_OK;
+}
+
+hipError_t launch_lm(uint32_t model, const Plan &plan, const FitArgs &args, hipStream_t s) {
+    switch (model) {
+    case 0: return launch_lm_model<0>(plan, args, s);
+    case 1: return launch_lm_model<1>(plan, args, s);
+    case 2: return launch_lm_model<2>(plan, args, s);
+    case 3: return launch_lm_model<3>(plan, args, s);
+    }
+    return hipErrorInvalidValue;
+}
+hipError_t launch_guess(uint32_t model, const GuessArgs &args, size_t lds, hipStream_t s) {
+    switch (model) {
+    case 0: return launch_guess_model<0>(args, lds, s);
+    case 1: return launch_guess_model<1>(args, lds, s);
+    case 2: return launch_guess_model<2>(args, lds, s);
+    case 3: return launch_guess_model<3>(args, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+bool options_valid(const bf_config &c, int p) {
+    if (!std::isfinite(c.initial_lambda) || c.initial_lambda <= 0.0f)
+        return false;
+    for (float tol : {c.gradient_tolerance, c.step_tolerance, c.cost_tolerance})
+        if (!std::isfinite(tol) || tol < 0.0f)
+            return false;
+    for (int k = 0; k < p; ++k)
+        if (std::isnan(c.lower[k]) || std::isnan(c.upper[k]) || c.lower[k] > c.upper[k] ||
+            c.lower[k] == INFINITY || c.upper[k] == -INFINITY)
+            return false;
+    return true;
+}
+
+bool shape_free(uint32_t model, const bf_config &c) {
+    bool free = c.lower[1] != c.upper[1] || c.lower[2] != c.upper[2];
+    if (model == BF_PSEUDO_VOIGT)
+        free = free || c.lower[3] != c.upper[3];
+    if (model == BF_TWO_LORENTZIAN)
+        free = free || c.lower[4] != c.upper[4] || c.lower[5] != c.upper[5];
+    return free;
+}
+
+float width_factor_of(uint32_t model) {
+    return model == BF_GAUSSIAN ? 2.354820045030949382f : model == BF_PSEUDO_VOIGT ? 1.0f : 2.0f;
+}
+
+// Stream-ordered scratch released at the end of the enqueuing call.
+struct Scratch {
+    hipStream_t stream;
+    void *ptrs[4] = {};
+    int used = 0;
+    explicit Scratch(hipStream_t s) : stream(s) {}
+    hipError_t get(void **out, size_t bytes) {
+        const hipError_t e = hipMallocAsync(out, bytes ? bytes : 1, stream);
+        if (e == hipSuccess)
+            ptrs[used++] = *out;
+        return e;
+    }
+    ~Scratch() {
+        for (int i = 0; i < used; ++i)
+            (void)hipFreeAsync(ptrs[i], stream);
+    }
+};
+
+int check_common(uint32_t model, const bf_config *cfg, char *err, size_t errlen) {
+    if (model > 3)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown fit model");
+    if (!cfg)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "config is required");
+    if (cfg->points == 0)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "expected x[M] and y[N,M] with M>0");
+    Runtime &rt = Runtime::get();
+    if (rt.status != BF_OK)
+        return fail(err, errlen, BF_UNAVAILABLE, "HIP fitting is unavailable: " + rt.why);
+    return BF_OK;
+}
+
+int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, const float *x,
+               const float *y, const float *p0, float *params, int32_t *status,
+               uint32_t *iterations, float *chi2, float *cov, float *se, int32_t *us, char *err,
+               size_t errlen) {
+    if (const int rc = check_common(model, cfg, err, errlen))
+        return rc;
+    const uint32_t n = cfg->fits, m = cfg->points;
+    const int P = kParams[model];
+    const bool unc = cov || se || us;
+    if (unc && !(cov && se && us))
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "covariance, stderr and uncertainty status are requested together");
+    if (n == 0)
+        return BF_OK;
+    if (!x || !y || !params || !status || !iterations || !chi2)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "x, y and every output must be non-null");
+    Plan plan;
+    if (const int rc = choose_plan(model, m, cfg->numerics, launch, plan, err, errlen))
+        return rc;
+    hipStream_t stream =
+        launch && launch->stream ? static_cast<hipStream_t>(launch->stream) : Runtime::get().stream;
+    const bool timing = g_timing.load(std::memory_order_relaxed);
+    Timing &tm = t_timing;
+    tm.recorded = false;
+    for (bool &b : tm.phase)
+        b = false;
+    Scratch scratch(stream);
+    GridInfo *grid = nullptr;
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&grid), sizeof(GridInfo)), "hipMallocAsync(grid)");
+    if (timing)
+        BF_TRY(record(0, stream), "hipEventRecord");
+    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, stream, x, m, grid);
+    BF_TRY(hipGetLastError(), "grid prologue launch");
+    if (timing)
+        BF_TRY(record(1, stream), "hipEventRecord");
+    tm.phase[0] = true;
+
+    const float *init = p0;
+    if (!init) {
+        const uint32_t count = m / 10 > 1 ? m / 10 : 1;
+        const uint32_t sort_n = uint32_t(pow2_ceil(2 * count));
+        const size_t lds = size_t(m + sort_n) * sizeof(float);
+        if (lds > 65536)
+            return fail(err, errlen, BF_INVALID_ARGUMENT,
+                        "the device initial guess supports M up to 13,000 points; pass p0");
+        float *guess = nullptr;
+        BF_TRY(scratch.get(reinterpret_cast<void **>(&guess), size_t(n) * P * sizeof(float)),
+               "hipMallocAsync(guess)");
+        GuessArgs ga{x, y, (long long)m, 1, grid, n, m, m, sort_n, guess, nullptr, nullptr};
+        BF_TRY(launch_guess(model, ga, lds, stream), "initial guess launch");
+        init = guess;
+        tm.phase[1] = true;
+    }
+    if (timing)
+        BF_TRY(record(2, stream), "hipEventRecord");
+
+    const float *ydata = y;
+    long long ys_fit = m, ys_pt = 1;
+    if (plan.G == 1 && launch && launch->point_major) {
+        float *yt = nullptr;
+        BF_TRY(scratch.get(reinterpret_cast<void **>(&yt), size_t(n) * m * sizeof(float)),
+               "hipMallocAsync(point-major y)");
+        hipLaunchKernelGGL(transpose_kernel, dim3((n + 63) / 64, (m + 63) / 64), dim3(256), 0,
+                           stream, y, yt, n, m);
+        BF_TRY(hipGetLastError(), "transpose launch");
+        ydata = yt;
+        ys_fit = 1;
+        ys_pt = n;
+        tm.phase[2] = true;
+    }
+    if (timing)
+        BF_TRY(record(3, stream), "hipEventRecord");
+
+    FitArgs a{};
+    a.x = x;
+    a.y = ydata;
+    a.ys_fit = ys_fit;
+    a.ys_pt = ys_pt;
+    a.p0 = init;
+    a.params = params;
+    a.status = status;
+    a.iterations = iterations;
+    a.chi2 = chi2;
+    a.cov = unc ? cov : nullptr;
+    a.stderr_ = se;
+    a.ustatus = us;
+    a.grid = grid;
+    a.n = n;
+    a.m = m;
+    a.max_iterations = cfg->max_iterations;
+    a.initial_lambda = cfg->initial_lambda;
+    a.gradient_tolerance = cfg->gradient_tolerance;
+    a.step_tolerance = cfg->step_tolerance;
+    a.cost_tolerance = cfg->cost_tolerance;
+    for (int k = 0; k < 8; ++k) {
+        a.lower[k] = cfg->lower[k];
+        a.upper[k] = cfg->upper[k];
+    }
+    a.numerics = cfg->numerics;
+    a.options_ok = options_valid(*cfg, P);
+    a.free_shape = shape_free(model, *cfg);
+    a.width_factor = width_factor_of(model);
+    BF_TRY(launch_lm(model, plan, a, stream), "LM kernel launch");
+    tm.phase[3] = true;
+    if (timing) {
+        BF_TRY(record(4, stream), "hipEventRecord");
+        tm.recorded = true;
+    }
+    return BF_OK;
+}
+
+}  // namespace
+}  // namespace bfhip
+
+using namespace bfhip;
+
+extern "C" {
+
+int bf_hip_abi_version(void) { return BF_HIP_ABI_VERSION; }
+
+int bf_hip_available(void) { return Runtime::get().status == BF_OK ? 1 : 0; }
+
+void bf_hip_default_config(bf_config *cfg) {
+    if (!cfg)
+        return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->max_iterations = 100;
+    cfg->initial_lambda = 1e-3f;
+    cfg->gradient_tolerance = 1e-5f;
+    cfg->step_tolerance = 1e-6f;
+    cfg->cost_tolerance = 1e-6f;
+    for (int k = 0; k < 8; ++k) {
+        cfg->lower[k] = -INFINITY;
+        cfg->upper[k] = INFINITY;
+    }
+}
+
+int bf_hip_set_timing(int enable) {
+    g_timing.store(enable != 0);
+    return BF_OK;
+}
+
+int bf_hip_last_timing(double *times, int count) {
+    Timing &tm = t_timing;
+    for (int i = 0; i < count; ++i)
+        times[i] = 0.0;
+    if (!tm.recorded)
+        return BF_INVALID_ARGUMENT;
+    if (hipEventSynchronize(tm.ev[4]) != hipSuccess)
+        return BF_RUNTIME_ERROR;
+    for (int i = 0; i < count && i < 4; ++i) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]) != hipSuccess)
+            return BF_RUNTIME_ERROR;
+        times[i] = ms;
+    }
+    return BF_OK;
+}
+
+int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch,
+                      const float *d_x, const float *d_y, const float *d_p0, float *d_params,
+                      int32_t *d_status, uint32_t *d_iterations, float *d_chi2, float *d_cov,
+                      float *d_stderr, int32_t *d_ustatus, char *err, size_t errlen) {
+    return fit_device(model, cfg, launch, d_x, d_y, d_p0, d_params, d_status, d_iterations,
+                      d_chi2, d_cov, d_stderr, d_ustatus, err, errlen);
+}
+
+int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, const float *x,
+               const float *y, const float *p0, float *params, int32_t *status,
+               uint32_t *iterations, float *chi2, float *cov, float *stderr_, int32_t *ustatus,
+               char *err, size_t errlen) {
+    if (const int rc = check_common(model, cfg, err, errlen))
+        return rc;
+    const bool unc = cov || stderr_ || ustatus;
+    if (unc && !(cov && stderr_ && ustatus))
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "covariance, stderr and uncertainty status are requested together");
+    const size_t n = cfg->fits, m = cfg->points, P = size_t(kParams[model]);
+    if (n == 0)
+        return BF_OK;
+    if (!x || !y || !params || !status || !iterations || !chi2)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "x, y and every output must be non-null");
+    Runtime &rt = Runtime::get();
+    std::lock_guard<std::mutex> lock(rt.host_calls);
+    bf_launch own{};
+    if (launch)
+        own = *launch;
+    own.stream = rt.stream;
+    hipStream_t s = rt.stream;
+    Scratch scratch(s);
+    float *dx, *dy, *dp0 = nullptr, *dparams, *dchi2, *dcov = nullptr, *dse = nullptr;
+    int32_t *dstatus, *dus = nullptr;
+    uint32_t *dit;
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dx), m * 4), "hipMallocAsync(x)");
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dy), n * m * 4), "hipMallocAsync(y)");
+    if (p0)
+        BF_TRY(scratch.get(reinterpret_cast<void **>(&dp0), n * P * 4), "hipMallocAsync(p0)");
+    // One output block: params | chi2 | status | iterations | cov | stderr | ustatus.
+    const size_t out_bytes = n * (P * 4 + 4 + 4 + 4) + (unc ? n * (P * P * 4 + P * 4 + 4) : 0);
+    char *dout = nullptr;
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dout), out_bytes), "hipMallocAsync(outputs)");
+    dparams = reinterpret_cast<float *>(dout);
+    dchi2 = dparams + n * P;
+    dstatus = reinterpret_cast<int32_t *>(dchi2 + n);
+    dit = reinterpret_cast<uint32_t *>(dstatus + n);
+    if (unc) {
+        dcov = reinterpret_cast<float *>(dit + n);
+        dse = dcov + n * P * P;
+        dus = reinterpret_cast<int32_t *>(dse + n * P);
+    }
+    BF_TRY(hipMemcpyAsync(dx, x, m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(x)");
+    BF_TRY(hipMemcpyAsync(dy, y, n * m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(y)");
+    if (p0)
+        BF_TRY(hipMemcpyAsync(dp0, p0, n * P * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(p0)");
+    const int rc = fit_device(model, cfg, &own, dx, dy, dp0, dparams, dstatus, dit, dchi2, dcov,
+                              dse, dus, err, errlen);
+    if (rc != BF_OK)
+        return rc;
+    BF_TRY(hipMemcpyAsync(params, dparams, n * P * 4, hipMemcpyDeviceToHost, s), "copy params");
+    BF_TRY(hipMemcpyAsync(chi2, dchi2, n * 4, hipMemcpyDeviceToHost, s), "copy chi2");
+    BF_TRY(hipMemcpyAsync(status, dstatus, n * 4, hipMemcpyDeviceToHost, s), "copy status");
+    BF_TRY(hipMemcpyAsync(iterations, dit, n * 4, hipMemcpyDeviceToHost, s), "copy iterations");
+    if (unc) {
+        BF_TRY(hipMemcpyAsync(cov, dcov, n * P * P * 4, hipMemcpyDeviceToHost, s), "copy cov");
+        BF_TRY(hipMemcpyAsync(stderr_, dse, n * P * 4, hipMemcpyDeviceToHost, s), "copy stderr");
+        BF_TRY(hipMemcpyAsync(ustatus, dus, n * 4, hipMemcpyDeviceToHost, s), "copy ustatus");
+    }
+    BF_TRY(hipStreamSynchronize(s), "fit completion");
+    return BF_OK;
+}
+
+int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const float *y,
+                 float *params, float *baseline, int32_t *gstatus, char *err, size_t errlen) {
+    bf_config cfg;
+    bf_hip_default_config(&cfg);
+    cfg.points = m;
+    cfg.fits = n;
+    if (const int rc = check_common(model, &cfg, err, errlen))
+        return rc;
+    if (n == 0)
+        return BF_OK;
+    if (!x || !y || !params || !baseline || !gstatus)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "every array must be non-null");
+    Runtime &rt = Runtime::get();
+    std::lock_guard<std::mutex> lock(rt.host_calls);
+    hipStream_t s = rt.stream;
+    const size_t P = size_t(kParams[model]);
+    const uint32_t count = m / 10 > 1 ? m / 10 : 1;
+    const uint32_t sort_n = uint32_t(pow2_ceil(2 * count));
+    const size_t lds = size_t(m + sort_n) * sizeof(float);
+    if (lds > 65536)
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "the device initial guess supports M up to 13,000 points");
+    Scratch scratch(s);
+    float *dx, *dy, *dp, *db;
+    int32_t *dg;
+    GridInfo *grid;
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dx), size_t(m) * 4), "hipMallocAsync");
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dy), size_t(n) * m * 4), "hipMallocAsync");
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&dp), size_t(n) * (P + 2) * 4), "hipMallocAsync");
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&grid), sizeof(GridInfo)), "hipMallocAsync");
+    db = dp + size_t(n) * P;
+    dg = reinterpret_cast<int32_t *>(db + n);
+    BF_TRY(hipMemcpyAsync(dx, x, size_t(m) * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    BF_TRY(hipMemcpyAsync(dy, y, size_t(n) * m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, s, dx, m, grid);
+    BF_TRY(hipGetLastError(), "grid prologue launch");
+    GuessArgs ga{dx, dy, (long long)m, 1, grid, n, m, m, sort_n, dp, db, dg};
+    BF_TRY(launch_guess(model, ga, lds, s), "initial guess launch");
+    BF_TRY(hipMemcpyAsync(params, dp, size_t(n) * P * 4, hipMemcpyDeviceToHost, s), "copy");
+    BF_TRY(hipMemcpyAsync(baseline, db, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");
+    BF_TRY(hipMemcpyAsync(gstatus, dg, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");
+    BF_TRY(hipStreamSynchronize(s), "guess completion");
+    return BF_OK;
+}
+
+int bf_hip_model_eval(uint32_t model, uint32_t n, const float *x, const float *params, float *value,
+                      float *jac, char *err, size_t errlen) {
+    bf_config cfg;
+    bf_hip_default_config(&cfg);
+    cfg.points = 1;
+    if (const int rc = check_common(model, &cfg, err, errlen))
+        return rc;
+    if (n == 0)
+        return BF_OK;
+    Runtime &rt = Runtime::get();
+    std::lock_guard<std::mutex> lock(rt.host_calls);
+    hipStream_t s = rt.stream;
+    const size_t P = size_t(kParams[model]);
+    Scratch scratch(s);
+    float *buf;
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&buf), size_t(n) * (1 + 8 + 1 + P) * 4),
+           "hipMallocAsync");
+    float *dx = buf, *dp = dx + n, *dv = dp + size_t(n) * 8, *dj = dv + n;
+    BF_TRY(hipMemcpyAsync(dx, x, size_t(n) * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    BF_TRY(hipMemcpyAsync(dp, params, size_t(n) * 32, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    EvalArgs ea{dx, dp, dv, dj, n};
+    hipError_t e = hipErrorInvalidValue;
+    switch (model) {
+    case 0: e = launch_eval_model<0>(ea, s); break;
+    case 1: e = launch_eval_model<1>(ea, s); break;
+    case 2: e = launch_eval_model<2>(ea, s); break;
+    case 3: e = launch_eval_model<3>(ea, s); break;
+    }
+    BF_TRY(e, "model evaluation launch");
+    BF_TRY(hipMemcpyAsync(value, dv, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");
+    BF_TRY(hipMemcpyAsync(jac, dj, size_t(n) * P * 4, hipMemcpyDeviceToHost, s), "copy");
+    BF_TRY(hipStreamSynchronize(s), "model evaluation completion");
+    return BF_OK;
+}
+
+int bf_hip_malloc(void **ptr, size_t bytes, char *err, size_t errlen) {
+    if (Runtime::get().status != BF_OK)
+        return fail(err, errlen, BF_UNAVAILABLE, "HIP is unavailable: " + Runtime::get().why);
+    BF_TRY(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+    return BF_OK;
+}
+
+int bf_hip_free(void *ptr) { return hipFree(ptr) == hipSuccess ? BF_OK : BF_RUNTIME_ERROR; }
+
+int bf_hip_memcpy(void *dst, const void *src, size_t bytes, int kind, char *err, size_t errlen) {
+    const hipMemcpyKind k = kind == 1   ? hipMemcpyHostToDevice
+                            : kind == 2 ? hipMemcpyDeviceToHost
+                                        : hipMemcpyDeviceToDevice;
+    BF_TRY(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+    return BF_OK;
+}
+
+int bf_hip_synchronize(char *err, size_t errlen) {
+    BF_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return BF_OK;
+}
+
+}  // extern "C"

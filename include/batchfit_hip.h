@@ -1,0 +1,135 @@
+/* batchfit_hip.h — C ABI of the MI355X (gfx950) batched bounded
+ * Levenberg–Marquardt fitter, libbatchfit_hip.so.
+ *
+ * This is the drop-in seam below the reference's GPU backend: it takes the
+ * place of metal::detail::run_gpu / submit_gpu (include/batchfit/metal/batch.hpp:90-92,
+ * src/metal/runtime.cpp:165-316) with the same inputs (shared x[M], one row of
+ * y per fit, initial parameters, the 108-byte Config of
+ * include/batchfit/metal/types.hpp:7-15) and the same outputs (params, status,
+ * iterations, chi2, optional covariance/stderr/uncertainty status). Callers:
+ * the C++ façade batchfit::fit (include/batchfit/fit.hpp) and the Python
+ * binding batchfit_amd.fit(..., device="hip").
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ types or exceptions cross the ABI.
+ *  - Every entry returns BF_OK (0) or an error code and writes a message into
+ *    err[errlen] when err is not NULL. Per-fit numerical problems are status
+ *    codes, never errors (DESIGN.md:52-54 of the reference).
+ *  - Status codes are cpu/solver.hpp:16-23; uncertainty status codes are
+ *    uncertainty_types.hpp:3-10; both int32. Iterations are uint32.
+ *  - y is fit-major (row i = fit i, M contiguous floats) unless stated.
+ *  - A NULL p0 runs the data-only initial guess on the device
+ *    (cpu/initial_guess.hpp:120-249, called by python/bindings.cpp:69-80).
+ */
+#ifndef BATCHFIT_HIP_H
+#define BATCHFIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BF_HIP_ABI_VERSION 1
+
+enum bf_error {
+    BF_OK = 0,
+    BF_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    BF_RUNTIME_ERROR = 2,    /* a HIP call failed; message names it */
+    BF_UNAVAILABLE = 3       /* no gfx950 device visible */
+};
+
+enum bf_model { BF_GAUSSIAN = 0, BF_LORENTZIAN = 1, BF_PSEUDO_VOIGT = 2, BF_TWO_LORENTZIAN = 3 };
+
+/* numerics bits (bit0/bit1 as Config::numerics, src/metal/lm.metal:8-10) */
+enum bf_numerics {
+    BF_COMPENSATED = 1u,    /* Kahan per-lane partials */
+    BF_SCALED_CHOLESKY = 2u,
+    BF_MIXED_FP64 = 4u      /* FP64 group combine, solve, accept test, covariance */
+};
+
+/* Byte layout of metal::detail::Config (include/batchfit/metal/types.hpp:7-15),
+ * 108 bytes. minimum_spacing, varied_x and width_factor are derived by the
+ * library from x and the model (batch.hpp:236-244) and ignored on input. */
+typedef struct bf_config {
+    uint32_t fits, points, max_iterations;
+    float initial_lambda, gradient_tolerance, step_tolerance, cost_tolerance;
+    float lower[8], upper[8];
+    uint32_t numerics;
+    float minimum_spacing;
+    uint32_t varied_x;
+    float width_factor;
+} bf_config;
+
+/* Fit-to-hardware mapping (replaces metal::KernelVariant, batch.hpp:19). */
+enum bf_mapping {
+    BF_MAP_AUTO = 0,  /* tuned table by (model, M) */
+    BF_MAP_LANE = 1,  /* one lane per fit, y streamed (point-major when transposed) */
+    BF_MAP_GROUP = 2  /* group_lanes lanes per fit, y held in registers */
+};
+
+typedef struct bf_launch {
+    uint32_t mapping;     /* bf_mapping */
+    uint32_t group_lanes; /* 16, 32 or 64 for BF_MAP_GROUP; 0 = tuned */
+    uint32_t point_major; /* BF_MAP_LANE only: 1 = transpose y on the device first */
+    uint32_t reserved;
+    void *stream;         /* hipStream_t; NULL = the library's stream */
+} bf_launch;
+
+/* Library and device facts. */
+int bf_hip_abi_version(void);
+/* 1 if a gfx950 device is usable (bindings.cpp:208-211 metal_available). */
+int bf_hip_available(void);
+/* Fill cfg with the reference defaults (cpu/solver.hpp:26-41, float): 100
+ * iterations, lambda 1e-3, tolerances 1e-5/1e-6/1e-6, unbounded, numerics 0. */
+void bf_hip_default_config(bf_config *cfg);
+
+/* Synchronous fit on host arrays (python/bindings.cpp:60-121 compute(),
+ * batchfit::fit_batch include/batchfit/metal/batch.hpp:268-284).
+ * x[M], y[N*M] fit-major, p0[N*P] or NULL; outputs params[N*P], status[N],
+ * iterations[N], chi2[N]; cov[N*P*P], stderr_[N*P], ustatus[N] all NULL or
+ * all non-NULL (uncertainties, cpu/uncertainty.hpp:13-110). launch may be NULL. */
+int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, const float *x,
+               const float *y, const float *p0, float *params, int32_t *status,
+               uint32_t *iterations, float *chi2, float *cov, float *stderr_, int32_t *ustatus,
+               char *err, size_t errlen);
+
+/* Asynchronous fit on device arrays already resident in HBM, enqueued on
+ * launch->stream (submit_batch, include/batchfit/metal/batch.hpp:253-264).
+ * Same arguments as bf_hip_fit with device pointers. Needs no host copy; its
+ * scratch is taken stream-ordered from the device pool. */
+int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch,
+                      const float *d_x, const float *d_y, const float *d_p0, float *d_params,
+                      int32_t *d_status, uint32_t *d_iterations, float *d_chi2, float *d_cov,
+                      float *d_stderr, int32_t *d_ustatus, char *err, size_t errlen);
+
+/* Data-only initial guess alone, host arrays (cpu/initial_guess.hpp:120-249):
+ * params[N*P], baseline[N], gstatus[N] (GuessStatus codes 0..3). */
+int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const float *y,
+                 float *params, float *baseline, int32_t *gstatus, char *err, size_t errlen);
+
+/* Device model evaluation for parity tests (tests/models_metal.mm:19-60):
+ * value[i] and jac[i*P..] of model at (x[i], params[i*8..]) for i < n. */
+int bf_hip_model_eval(uint32_t model, uint32_t n, const float *x, const float *params,
+                      float *value, float *jac, char *err, size_t errlen);
+
+/* Enable (1) or disable (0) kernel timing with HIP events; also enabled by the
+ * environment variable BF_HIP_TIMING. */
+int bf_hip_set_timing(int enable);
+/* Kernel timing of the most recent call on this thread, in milliseconds,
+ * measured with HIP events on the stream the kernels ran on:
+ * times[0] grid prologue, [1] guess, [2] transpose, [3] LM (+covariance). */
+int bf_hip_last_timing(double *times, int count);
+
+/* Device memory helpers so hosts without another HIP binding can stage
+ * inputs in HBM: plain hipMalloc/hipFree/hipMemcpy (kind 1 H2D, 2 D2H, 3 D2D). */
+int bf_hip_malloc(void **ptr, size_t bytes, char *err, size_t errlen);
+int bf_hip_free(void *ptr);
+int bf_hip_memcpy(void *dst, const void *src, size_t bytes, int kind, char *err, size_t errlen);
+int bf_hip_synchronize(char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

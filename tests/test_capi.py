@@ -1,0 +1,98 @@
+"""CPU checks of the C ABI and the Python boundary: the library loads, exports
+every symbol include/batchfit_hip.h declares, keeps the 108-byte Config
+layout, and rejects bad arguments before any device work — with the
+reference's exception types (python/tests/test_api.py:78-101)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import batchfit_amd
+from batchfit_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "batchfit_hip.h")).read()
+    return sorted(set(re.findall(r"\b(bf_hip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.library()
+    names = declared_symbols()
+    assert names, "no declarations parsed"
+    assert sorted(_native.EXPORTS) == names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.bf_hip_abi_version() == 1
+    assert lib.bf_hip_available() in (0, 1)
+
+
+def test_config_layout_and_defaults():
+    assert ctypes.sizeof(_native.Config) == 108
+    offsets = {name: getattr(_native.Config, name).offset for name, _ in _native.Config._fields_}
+    # include/batchfit/metal/types.hpp:7-15 of the reference: 4-byte scalars in order.
+    assert offsets["lower"] == 28 and offsets["upper"] == 60 and offsets["numerics"] == 92
+    assert offsets["width_factor"] == 104
+    cfg = _native.default_config()
+    assert cfg.max_iterations == 100
+    assert cfg.initial_lambda == pytest.approx(1e-3)
+    assert (cfg.gradient_tolerance, cfg.step_tolerance, cfg.cost_tolerance) == pytest.approx(
+        (1e-5, 1e-6, 1e-6))
+    assert all(np.isinf(cfg.lower[k]) and cfg.lower[k] < 0 for k in range(8))
+    assert cfg.numerics == 0
+
+
+def test_abi_rejects_bad_arguments_without_a_device():
+    lib = _native.library()
+    cfg = _native.default_config()
+    err = _native.error_buffer()
+    assert lib.bf_hip_fit(7, ctypes.byref(cfg), None, None, None, None, None, None, None, None,
+                          None, None, None, err, len(err)) == _native.BF_INVALID_ARGUMENT
+    assert b"unknown fit model" in err.value
+    cfg.points = 0
+    assert lib.bf_hip_fit(1, ctypes.byref(cfg), None, None, None, None, None, None, None, None,
+                          None, None, None, err, len(err)) == _native.BF_INVALID_ARGUMENT
+    assert b"M>0" in err.value
+    assert lib.bf_hip_fit(1, None, None, None, None, None, None, None, None, None, None, None,
+                          None, err, len(err)) == _native.BF_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("change,error", [
+    ({"model": "missing"}, ValueError), ({"device": "gpu"}, ValueError),
+    ({"device": "cpu"}, ValueError), ({"p0": [1, 2]}, ValueError),
+    ({"bounds": ([0, 0], [1, 1])}, ValueError), ({"bounds": (3, 2)}, ValueError),
+    ({"bounds": (np.nan, np.inf)}, ValueError), ({"bounds": (-np.inf, 1e300)}, ValueError),
+    ({"mapping": "warp"}, ValueError)])
+def test_invalid_options(change, error):
+    x = np.linspace(-4, 4, 129)
+    y = np.tile(2 / (1 + ((x - .2) / .7) ** 2), (8, 1))
+    with pytest.raises(error):
+        batchfit_amd.fit(x, y, **{"device": "hip", **change})
+
+
+@pytest.mark.parametrize("x,y,error", [
+    (np.zeros((2, 2)), np.zeros((2, 2)), ValueError), ([], [], ValueError),
+    ([1, 2], [[1, 2, 3]], ValueError), ([1j, 2j], [[1, 2]], TypeError),
+    ([1, 2], [["a", "b"]], TypeError)])
+def test_invalid_arrays(x, y, error):
+    with pytest.raises(error):
+        batchfit_amd.fit(x, y, device="hip")
+
+
+def test_uncertainty_flag_is_explicit_boolean():
+    with pytest.raises(TypeError, match="boolean"):
+        batchfit_amd.fit([0, 1, 2], [1, 2, 1], uncertainties="yes")
+
+
+def test_no_oracle_in_product_path():
+    """The product package never imports the CPU oracle."""
+    pkg = os.path.join(ROOT, "batchfit_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f), errors="replace").read()
+                assert "oracle" not in text.lower(), f

@@ -1,0 +1,304 @@
+"""GPU parity of the HIP path against the CPU oracle on identical inputs.
+
+Tolerances (written here, per test):
+  * vs the FP32 oracle: the reference's own GPU gate (tests/metal_lm_checks.hpp:41-45)
+    params |gpu−cpu| ≤ 2e-4 + 2e-4·|cpu|, χ² ≤ 1e-6 + 2e-5·|cpu|, equal status.
+  * vs the FP64 oracle on the metric workload: status agreement ≥ 99.9 %;
+    converged rows amplitude and width relative error ≤ 1e-4, centre error
+    ≤ 1e-4·width, χ² relative error ≤ 1e-5.
+  * the data-only guess is integer/branch work on FP32 values and must equal
+    the FP32 oracle bit for bit.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import datasets
+
+pytestmark = pytest.mark.gpu
+
+MAPPINGS = ["lane", "group16", "group32", "group64"]
+NUMERICS = [0, 5]  # FP32 as the reference GPU kernels; compensated FP32 + FP64 solve
+P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
+
+
+def assert_fp32_gate(got, ref, label):
+    bad = np.nonzero(got.status != ref.status)[0]
+    assert bad.size == 0, f"{label}: status differs at rows {bad[:10]} " \
+                          f"gpu={got.status[bad[:10]]} cpu={ref.status[bad[:10]]}"
+    ok = ref.status == 0
+    d = np.abs(got.params[ok] - ref.params[ok]) - (2e-4 + 2e-4 * np.abs(ref.params[ok]))
+    assert np.all(d <= 0), f"{label}: params off by {d.max():.3g} beyond tolerance"
+    dc = np.abs(got.chi2[ok] - ref.chi2[ok]) - (1e-6 + 2e-5 * np.abs(ref.chi2[ok]))
+    assert np.all(dc <= 0), f"{label}: chi2 off by {dc.max():.3g} beyond tolerance"
+    assert np.all(got.iterations <= 100)
+
+
+def test_model_values_and_jacobians(hip, oracle_port):
+    """Device FP32 model math vs the float64 oracle (tests/models_metal.mm:19-60)."""
+    from batchfit_amd import _native
+    lib = _native.library()
+    rng = np.random.default_rng(17)
+    for model, p in P.items():
+        n = 2048
+        x = rng.uniform(-6, 6, n).astype(np.float32)
+        q = np.zeros((n, 8), np.float32)
+        q[:, 0] = rng.uniform(-5, 5, n)
+        q[:, 1] = rng.uniform(-4, 4, n)
+        q[:, 2] = rng.uniform(0.3, 3, n)
+        q[:, 3] = rng.uniform(-5, 5, n) if model == "two_lorentzian" else rng.uniform(0, 1, n)
+        q[:, 4] = rng.uniform(-4, 4, n)
+        q[:, 5] = rng.uniform(0.3, 3, n)
+        q[:, 6] = rng.uniform(-2, 2, n)
+        q[:, 7] = rng.uniform(-1, 1, n)
+        value = np.empty(n, np.float32)
+        jac = np.empty((n, p), np.float32)
+        err = _native.error_buffer()
+        _native.check(lib.bf_hip_model_eval(_native.MODELS[model], n, x.ctypes.data,
+                                            q.ctypes.data, value.ctypes.data, jac.ctypes.data,
+                                            err, len(err)), err)
+        for i in range(n):
+            rv, rj = oracle_port.model(model, float(x[i]), q[i].astype(np.float64))
+            assert abs(value[i] - rv) <= 4e-6 * (1 + abs(rv)), (model, i, value[i], rv)
+            assert np.all(np.abs(jac[i] - rj) <= 1e-5 * (1 + np.abs(rj))), (model, i)
+
+
+def test_guess_bitwise_config1(hip, oracle_port):
+    from batchfit_amd import _native
+    x, y, _ = datasets.config1(np.float32)
+    got = device_guess(x, y, "lorentzian")
+    ref = oracle_port.guess(x, y, "lorentzian")
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8))
+
+
+def device_guess(x, y, model):
+    from batchfit_amd import _native
+    lib = _native.library()
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    n, m = y.shape
+    params = np.empty((n, P[model]), np.float32)
+    baseline = np.empty(n, np.float32)
+    status = np.empty(n, np.int32)
+    err = _native.error_buffer()
+    _native.check(lib.bf_hip_guess(_native.MODELS[model], n, m, x.ctypes.data, y.ctypes.data,
+                                   params.ctypes.data, baseline.ctypes.data, status.ctypes.data,
+                                   err, len(err)), err)
+    return params, baseline, status
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("m,n", [(32, 257), (129, 257), (1024, 65)])
+def test_guess_bitwise_sets(hip, oracle_port, model, m, n):
+    x, y, _, _, _ = datasets.comparison_set(model, n, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    got = device_guess(x, y, model)
+    ref = oracle_port.guess(x, y, model)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8)), model
+
+
+def test_guess_bitwise_metric_and_doublets(hip, oracle_port):
+    x, y, _, _ = datasets.config2(n=1 << 20, rows=4096)
+    got = device_guess(x, y, "pseudo_voigt")
+    ref = oracle_port.guess(x, y, "pseudo_voigt", threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8))
+    x3, y3, _, _, _, _ = datasets.config3(n=262144, rows=1024)
+    got = device_guess(x3, y3, "two_lorentzian")
+    ref = oracle_port.guess(x3, y3, "two_lorentzian", threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8))
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("m,n", [(32, 257), (129, 257), (1024, 65)])
+def test_fit_matches_fp32_oracle(hip, oracle_port, model, m, n):
+    x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
+    for mapping in MAPPINGS:
+        for numerics in NUMERICS:
+            got = hip.fit(x, y, model, p0, bounds, "hip", mapping=mapping, numerics=numerics)
+            assert got.params.dtype == np.float32 and got.status.dtype == np.int32
+            assert got.iterations.dtype == np.uint32
+            assert_fp32_gate(got, ref, f"{model} M={m} {mapping} numerics={numerics}")
+
+
+@pytest.mark.parametrize("model", list(P))
+def test_fit_from_device_guess(hip, oracle_port, model):
+    x, y, _, bounds, _ = datasets.comparison_set(model, 257, 129)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    ref = oracle_port.fit(x, y, model, None, bounds, threads=8)
+    for numerics in NUMERICS:
+        got = hip.fit(x, y, model, None, bounds, "hip", numerics=numerics)
+        assert_fp32_gate(got, ref, f"{model} guessed numerics={numerics}")
+
+
+def test_metric_workload_vs_fp64_oracle(hip, oracle_port):
+    """16,384 rows of the pseudo-Voigt M=1024 workload, production numerics,
+    against the float64 oracle on the same float32 inputs."""
+    x, y, bounds, _ = datasets.config2(rows=16384)
+    ref = oracle_port.fit(x.astype(np.float64), y.astype(np.float64), "pseudo_voigt", None,
+                          bounds, threads=8)
+    got = hip.fit(x, y, "pseudo_voigt", None, bounds, "hip")
+    agree = got.status == ref.status
+    rate = agree.mean()
+    for i in np.nonzero(~agree)[0][:20]:
+        print(f"status disagreement row {i}: gpu={got.status[i]} fp64={ref.status[i]} "
+              f"iterations {got.iterations[i]}/{ref.iterations[i]} params {got.params[i]} "
+              f"vs {ref.params[i]}")
+    assert rate >= 0.999, rate
+    both = (got.status == 0) & (ref.status == 0)
+    g, r = got.params[both].astype(np.float64), ref.params[both]
+    amp = np.abs(g[:, 0] - r[:, 0]) / np.abs(r[:, 0])
+    width = np.abs(g[:, 2] - r[:, 2]) / np.abs(r[:, 2])
+    center = np.abs(g[:, 1] - r[:, 1]) / np.abs(r[:, 2])
+    chi = np.abs(got.chi2[both] - ref.chi2[both]) / ref.chi2[both]
+    print(f"FP64 errors over {both.sum()} converged rows: amplitude max {amp.max():.3g} "
+          f"rms {np.sqrt(np.mean(amp**2)):.3g}; width max {width.max():.3g}; centre/width max "
+          f"{center.max():.3g}; chi2 max {chi.max():.3g}; status agreement {rate:.5f}")
+    assert amp.max() <= 1e-4 and width.max() <= 1e-4 and center.max() <= 1e-4
+    assert chi.max() <= 1e-5
+
+
+@pytest.mark.parametrize("mapping", MAPPINGS)
+@pytest.mark.parametrize("numerics", NUMERICS)
+def test_failure_isolation(hip, golden, mapping, numerics):
+    """tests/metal_lm_checks.hpp:59-128 and cooperative_checks.hpp:203-257."""
+    g = golden["failures"]
+    x, y, p0 = datasets.isolation_fixture()
+    kw = dict(mapping=mapping, numerics=numerics)
+    r = hip.fit(x, y, "lorentzian", p0, None, "hip", **kw)
+    assert list(r.status) == [2, 0, 3] and list(r.iterations) == [1, 0, 0]
+    assert r.chi2[0] == 3 and np.isnan(r.chi2[2])
+    np.testing.assert_array_equal(r.params, g["iso_0_params"])
+    r = hip.fit(x, y, "lorentzian", p0, None, "hip", max_iterations=0, **kw)
+    assert r.status[0] == 1 and r.chi2[0] == 3
+    r = hip.fit(x, y, "lorentzian", p0, ([-np.inf, 0, 1], [np.inf, 0, 1]), "hip", **kw)
+    assert r.status[0] == 0 and r.params[0, 1] == 0 and r.params[0, 2] == 1
+    r = hip.fit(x, y, "lorentzian", p0, ([-np.inf, 0, 1], [1.5, 0, 1]), "hip", **kw)
+    assert np.all(r.status == 0) and np.all(r.params[:, 0] == np.float32(1.5))
+    bad_x = x.copy()
+    bad_x[0] = np.nan
+    assert np.all(hip.fit(bad_x, y, "lorentzian", p0, None, "hip", **kw).status == 3)
+    bad_y = y.copy()
+    bad_y[1, 1] = np.nan
+    r = hip.fit(x, bad_y, "lorentzian", p0, None, "hip", **kw)
+    assert r.status[0] == 2 and r.status[1] == 3
+    empty = hip.fit(np.linspace(0, 1, 32), np.empty((0, 32)), "gaussian", None, None, "hip", **kw)
+    assert empty.params.shape == (0, 3) and empty.status.shape == (0,)
+
+
+@pytest.mark.parametrize("model", list(P))
+def test_pathology_matrix(hip, golden, model):
+    """tests/numerics_checks.hpp:280-350: degenerate, non-finite, edge,
+    sub-spacing, ordinary and overflow rows stay isolated in every form."""
+    g = golden["failures"]
+    x, inits, _ = datasets.pathology_fixture(model)
+    y = g[f"path_{model}_y"]
+    for mapping in MAPPINGS:
+        for numerics in (0, 1, 2, 3, 5, 7):
+            if mapping in ("group16", "group32") and numerics not in (0, 5):
+                continue
+            r = hip.fit(x, y, model, inits, None, "hip", mapping=mapping, numerics=numerics)
+            assert list(r.status) == [4, 4, 3, 3, 3, 0, 5, 0, 3], (mapping, numerics, r.status)
+            assert np.all(np.isnan(r.chi2[2:5]))
+            for i in (0, 1, 5, 6, 7):
+                assert np.isfinite(r.chi2[i]) and np.all(np.isfinite(r.params[i]))
+    bad_x = x.copy()
+    bad_x[17] = np.nan
+    for mapping in MAPPINGS:
+        assert np.all(hip.fit(bad_x, y, model, inits, None, "hip", mapping=mapping).status == 3)
+
+
+def test_row_isolation_api(hip):
+    """python/tests/test_api.py:69-76."""
+    x, y, p0 = datasets.api_isolation_fixture(np.float32)
+    r = hip.fit(x, y, "lorentzian", p0[0], None, "hip")
+    np.testing.assert_array_equal(r.status, [0, 3, 3, 4, 0])
+    c = hip.fit(x, y[[0]], "lorentzian", p0[0], ([0, .2, .7], [np.inf, .2, .7]), "hip")
+    np.testing.assert_allclose(c.params[0], [2, .2, .7], atol=2e-4)
+
+
+@pytest.mark.parametrize("model", list(P))
+def test_uncertainties_match_oracle(hip, oracle_port, model):
+    """tests/uncertainty_checks.hpp:104-190: covariance at the accepted point
+    within 1e-9 + 2e-4·|cpu| of the FP32 oracle applied to the GPU's own fit."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 64, 129)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    for mapping in MAPPINGS:
+        basic = hip.fit(x, y, model, p0, bounds, "hip", mapping=mapping)
+        full = hip.fit(x, y, model, p0, bounds, "hip", mapping=mapping, uncertainties=True)
+        for a, b in zip(basic, full[:4]):
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                                  np.ascontiguousarray(b).view(np.uint8)), "fit bytes changed"
+        for i in range(64):
+            cov, se, us = oracle_port.uncertainty(model, x, full.params[i], int(full.status[i]),
+                                                  float(full.chi2[i]), bounds, dtype=np.float32)
+            assert full.uncertainty_status[i] == us, (mapping, i)
+            if us == 0:
+                np.testing.assert_allclose(full.covariance[i], cov, rtol=2e-4, atol=1e-9)
+                np.testing.assert_allclose(full.stderr[i], se, rtol=2e-4, atol=1e-8)
+        np.testing.assert_array_equal(full.covariance, full.covariance.transpose(0, 2, 1))
+
+
+def test_uncertainty_statuses(hip):
+    """python/tests/test_uncertainty.py:67-98."""
+    x = np.linspace(-5, 5, 129)
+    y = datasets.lorentzian(x, 2, .2, .7)
+    initial = [2, .2, .7]
+    failed = y.copy()
+    failed[17] = np.nan
+    r = hip.fit(x, failed, p0=initial, device="hip", uncertainties=True)
+    assert r.status[0] == 3 and r.uncertainty_status[0] == 1
+    assert np.all(np.isnan(r.covariance)) and np.all(np.isnan(r.stderr))
+    small_x = np.array([-1., 0., 1.])
+    small = hip.fit(small_x, datasets.lorentzian(small_x, 2, 0, 2), p0=[2, 0, 2], device="hip",
+                    uncertainties=True)
+    assert small.uncertainty_status[0] == 2
+    singular = hip.fit(np.zeros(20), np.full(20, 2.), p0=[2, 0, 1], device="hip",
+                       uncertainties=True)
+    assert singular.status[0] == 0 and singular.uncertainty_status[0] == 3
+    boundary = hip.fit(x, y, p0=initial, bounds=([2, -5, .01], [np.inf, 5, 5]), device="hip",
+                       uncertainties=True)
+    assert boundary.uncertainty_status[0] == 4
+    noisy = y + np.random.default_rng(101).normal(0, .03, len(x))
+    fixed = hip.fit(x, noisy, p0=initial, bounds=([0, .2, .7], [np.inf, .2, .7]), device="hip",
+                    uncertainties=True)
+    assert fixed.uncertainty_status[0] == 0
+    np.testing.assert_array_equal(fixed.covariance[0, 1:], 0)
+    np.testing.assert_array_equal(fixed.stderr[0, 1:], 0)
+    shape = datasets.lorentzian(x, 1, .2, .7)
+    expected = fixed.chi2[0] / (len(x) - 1) / np.sum(shape ** 2)
+    np.testing.assert_allclose(fixed.covariance[0, 0, 0], expected, rtol=1e-4, atol=1e-12)
+    empty = hip.fit(x, np.empty((0, len(x))), device="hip", uncertainties=True)
+    assert empty.covariance.shape == (0, 3, 3) and empty.stderr.shape == (0, 3)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 255, 256, 257])
+def test_batch_tails(hip, oracle_port, n):
+    x, y, p0, bounds, _ = datasets.comparison_set("lorentzian", 257, 129)
+    x, y = x.astype(np.float32), y[:n].astype(np.float32)
+    ref = oracle_port.fit(x, y, "lorentzian", p0, bounds)
+    for mapping in MAPPINGS:
+        assert_fp32_gate(hip.fit(x, y, "lorentzian", p0, bounds, "hip", mapping=mapping), ref,
+                         f"N={n} {mapping}")
+
+
+@pytest.mark.parametrize("m", [5, 31, 33, 64, 65, 2047, 2049, 4096])
+def test_point_count_edges(hip, oracle_port, m):
+    """Register tiles of every size, the partial last tile and the streamed form."""
+    x, y, p0, bounds, _ = datasets.comparison_set("pseudo_voigt", 48, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    ref = oracle_port.fit(x, y, "pseudo_voigt", p0, bounds, threads=8)
+    maps = MAPPINGS if m <= 1024 else ["lane", "group64"]
+    for mapping in maps:
+        assert_fp32_gate(hip.fit(x, y, "pseudo_voigt", p0, bounds, "hip", mapping=mapping), ref,
+                         f"M={m} {mapping}")
