@@ -24,6 +24,7 @@ EXPORTS = (
     "bf_hip_abi_version", "bf_hip_available", "bf_hip_default_config", "bf_hip_fit",
     "bf_hip_fit_device", "bf_hip_guess", "bf_hip_model_eval", "bf_hip_set_timing",
     "bf_hip_last_timing", "bf_hip_malloc", "bf_hip_free", "bf_hip_memcpy", "bf_hip_synchronize",
+    "bf_hip_ragged_create", "bf_hip_ragged_run", "bf_hip_ragged_timing", "bf_hip_ragged_destroy",
 )
 
 

@@ -42,17 +42,83 @@ template <int MODEL> __global__ void __launch_bounds__(256) model_eval_kernel(Ev
         a.jac[size_t(i) * P + k] = jac[k];
 }
 
+// Shared-grid facts (cpu/solver.hpp:58-69, batch.hpp:236-240,
+// initial_guess.hpp:131-153) for one 256-thread block. Minima are exact, so the
+// parallel order gives the reference's sequential values.
+__device__ inline void grid_block(const float *x, uint32_t m, GridInfo *out) {
+    __shared__ float s_abs[256], s_gap[256];
+    __shared__ int s_flags[256];
+    const int t = threadIdx.x;
+    const float x0 = x[0];
+    float amin = __uint_as_float(0x7f800000u), gmin = 3.40282346638528859812e+38f;
+    int fl = 0;  // 1 non-finite, 2 not rising, 4 not falling, 8 varied, 16 not strictly rising
+    for (uint32_t j = t; j < m; j += 256) {
+        const float xj = x[j];
+        fl |= finite_f(xj) ? 0 : 1;
+        fl |= (xj != x0) ? 8 : 0;
+        if (j > 0) {
+            const float prev = x[j - 1];
+            const float d = xj - prev;
+            fl |= (d >= 0.0f) ? 0 : 2;
+            fl |= (d <= 0.0f) ? 0 : 4;
+            if (d != 0.0f)
+                amin = fminf(amin, fabsf(d));
+            fl |= (xj <= prev) ? 16 : 0;
+            gmin = fminf(gmin, d);
+        }
+    }
+    s_abs[t] = amin;
+    s_gap[t] = gmin;
+    s_flags[t] = fl;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            s_abs[t] = fminf(s_abs[t], s_abs[t + o]);
+            s_gap[t] = fminf(s_gap[t], s_gap[t + o]);
+            s_flags[t] |= s_flags[t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int f = s_flags[0];
+        const bool monotone = !(f & 2) || !(f & 4);
+        GridInfo g;
+        g.min_spacing = monotone && finite_f(s_abs[0]) ? s_abs[0] : 0.0f;
+        g.varied_x = (f & 8) ? 1 : 0;
+        g.x_finite = (f & 1) ? 0 : 1;
+        g.guess_grid_ok = !(f & 1) && !(f & 16) && finite_f(x[m - 1] - x[0]);
+        g.guess_spacing = s_gap[0];
+        const uint32_t count = m / 10 > 1 ? m / 10 : 1;
+        g.count = int(count);
+        g.left_x = std_midpoint(x[(count - 1) / 2], x[count / 2]);
+        g.right_x = std_midpoint(x[m - count + (count - 1) / 2], x[m - count + count / 2]);
+        *out = g;
+    }
+}
+
 template <int MODEL> hipError_t launch_lm_model(const Plan &plan, const FitArgs &args,
                                                 hipStream_t stream);
+// One (model, lanes-per-fit) class; each lives in its own translation unit
+// (bf_lm_<model>_g<G>.hip) so the library compiles in parallel.
+template <int MODEL, int G> hipError_t launch_lm_group(const Plan &plan, const FitArgs &args,
+                                                       hipStream_t stream);
 template <int MODEL> hipError_t launch_guess_model(const GuessArgs &args, size_t lds_bytes,
                                                    hipStream_t stream);
 template <int MODEL> hipError_t launch_eval_model(const EvalArgs &args, hipStream_t stream);
 
+// Tuned lanes per fit by (model, M): 16-lane groups for short spectra, whole
+// waves beyond (the same table drives uniform and ragged batches, so a fit's
+// arithmetic does not depend on how it was batched).
+inline int tuned_group(int model, uint32_t m) {
+    (void)model;
+    return m <= 256 ? 16 : 64;
+}
+
 // Instantiated plans: lane-per-fit with every numerics form; 64-lane groups
-// with tiles 0 (stream), 1 … 32 and every numerics form; 16- and 32-lane
+// with tiles 0 (stream), 2 … 32 and every numerics form; 16- and 32-lane
 // groups with tiles 2 … 32 in the plain FP32 and compensated-mixed forms.
 constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
-    const bool tile = R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || R == 32;
+    const bool tile = R == 2 || R == 4 || R == 8 || R == 16 || R == 32;
     if (G == 1)
         return R == 0;
     if (G == 64)

@@ -86,23 +86,26 @@ __host__ __device__ constexpr int tri(int k, int l) { return k * (k + 1) / 2 + l
 // non-finite entry or a non-finite solution.
 template <typename S, int P>
 __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
-    S d[P];
+    S d[P], dinv[P];
     if (scaled) {
 #pragma unroll
         for (int i = 0; i < P; ++i) {
             if (!finite_s(a[tri(i, i)]) || a[tri(i, i)] <= S(0))
                 return false;
             d[i] = sqrt(a[tri(i, i)]);
+            dinv[i] = S(1) / d[i];
         }
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            b[i] /= d[i];
+            b[i] *= dinv[i];
 #pragma unroll
             for (int j = 0; j <= i; ++j)
-                a[tri(i, j)] = (a[tri(i, j)] / d[i]) / d[j];
+                a[tri(i, j)] = (a[tri(i, j)] * dinv[i]) * dinv[j];
         }
     }
+    // Factor and forward substitution; one reciprocal per pivot.
     bool ok = true;
+    S inv[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
 #pragma unroll
@@ -111,14 +114,20 @@ __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
 #pragma unroll
             for (int k = 0; k < j; ++k)
                 v -= a[tri(i, k)] * a[tri(j, k)];
-            ok = ok && finite_s(v) && !(i == j && v <= S(0));
-            a[tri(i, j)] = i == j ? sqrt(v) : v / a[tri(j, j)];
+            if (i == j) {
+                ok = ok && finite_s(v) && v > S(0);
+                a[tri(i, i)] = sqrt(v);
+                inv[i] = S(1) / a[tri(i, i)];
+            } else {
+                ok = ok && finite_s(v);
+                a[tri(i, j)] = v * inv[j];
+            }
         }
         S v = b[i];
 #pragma unroll
         for (int j = 0; j < i; ++j)
             v -= a[tri(i, j)] * b[j];
-        b[i] = v / a[tri(i, i)];
+        b[i] = v * inv[i];
     }
 #pragma unroll
     for (int i = P - 1; i >= 0; --i) {
@@ -126,36 +135,43 @@ __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
 #pragma unroll
         for (int j = i + 1; j < P; ++j)
             v -= a[tri(j, i)] * b[j];
-        b[i] = v / a[tri(i, i)];
+        b[i] = v * inv[i];
         ok = ok && finite_s(b[i]);
     }
     if (scaled)
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            b[i] /= d[i];
+            b[i] *= dinv[i];
             ok = ok && finite_s(b[i]);
         }
     return ok;
 }
 
-// Per-lane partial accumulation: FMA (plain) or Kahan (compensated).
+// Per-lane partial accumulation. Plain: one FMA per term and point.
+// Compensated: FMA into a block partial over 4 points, then the block partial
+// is added into a Kahan pair (sum, carry) (numerics.hpp:12-18), so the
+// compensation costs 4 VALU per term per block instead of per point. A lane's
+// partial is (sum − carry).
 template <bool COMP> struct Acc;
 template <> struct Acc<false> {
     float s;
     __device__ __forceinline__ void zero() { s = 0.0f; }
     __device__ __forceinline__ void add_prod(float a, float b) { s = __builtin_fmaf(a, b, s); }
+    __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ float hi() const { return s; }
     __device__ __forceinline__ float lo() const { return 0.0f; }
 };
 template <> struct Acc<true> {
-    float s, c;  // true sum ≈ s − c
-    __device__ __forceinline__ void zero() { s = c = 0.0f; }
-    __device__ __forceinline__ void add_prod(float a, float b) {
+    float s, c, b;
+    __device__ __forceinline__ void zero() { s = c = b = 0.0f; }
+    __device__ __forceinline__ void add_prod(float x, float y) { b = __builtin_fmaf(x, y, b); }
+    __device__ __forceinline__ void flush() {
 #pragma clang fp contract(off)
-        const float adjusted = a * b - c;
+        const float adjusted = b - c;
         const float next = s + adjusted;
         c = (next - s) - adjusted;
         s = next;
+        b = 0.0f;
     }
     __device__ __forceinline__ float hi() const { return s; }
     __device__ __forceinline__ float lo() const { return c; }
@@ -185,8 +201,10 @@ __device__ __forceinline__ S reduce_term(const Acc<COMP> &a) {
 
 // Points of one fit as seen by one lane: a register tile (R > 0) or a
 // strided stream from memory (R == 0).
+// With a register tile, x is read from the block's LDS copy of the shared
+// grid (every fit of a launch shares it), which keeps R VGPRs free.
 template <int G, int R> struct Points {
-    float xv[R > 0 ? R : 1], yv[R > 0 ? R : 1];
+    float yv[R > 0 ? R : 1];
     const float *x, *y;
     long long ys_pt;
     int gl, m;
@@ -201,23 +219,54 @@ template <int G, int R> struct Points {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = gl + G * r;
-                const bool in = j < m;
-                xv[r] = in ? x[j] : 0.0f;
-                yv[r] = in ? y[j * ys_pt] : 0.0f;
+                yv[r] = j < m ? y[j * ys_pt] : 0.0f;
             }
         }
     }
-    // f(j, x, y) for every point this lane owns.
-    template <typename F> __device__ __forceinline__ void each(F &&f) const {
+    // f(x, y) for every point this lane owns, then flush() after every block
+    // of kBlock points. Slots below m/G are full for every lane of the group
+    // (a scalar branch); only slot m/G is partial.
+    static constexpr int kBlock = 4;
+    template <typename F, typename Flush>
+    __device__ __forceinline__ void each(F &&f, Flush &&flush) const {
         if constexpr (R > 0) {
+            if (m == G * R) {
+                // Every slot full: straight-line code with no joins.
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (gl + G * r < m)
-                    f(xv[r], yv[r]);
+                for (int r = 0; r < R; ++r) {
+                    f(x[gl + G * r], yv[r]);
+                    if (r % kBlock == kBlock - 1 || r == R - 1)
+                        flush();
+                }
+                return;
+            }
+            // Blocks that hold no point of the group are skipped entirely, so
+            // the arithmetic of a fit depends on (G, M) and not on the tile R.
+            const int full = m / G;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r < full)
+                    f(x[gl + G * r], yv[r]);
+                else if (r == full && gl + G * r < m)
+                    f(x[gl + G * r], yv[r]);
+                if ((r % kBlock == kBlock - 1 || r == R - 1) && G * (r - r % kBlock) < m)
+                    flush();
+            }
         } else {
-            for (int j = gl; j < m; j += G)
+            int k = 0;
+            for (int j = gl; j < m; j += G) {
                 f(x[j], y[j * ys_pt]);
+                if (++k == kBlock) {
+                    flush();
+                    k = 0;
+                }
+            }
+            if (k)
+                flush();
         }
+    }
+    template <typename F> __device__ __forceinline__ void each(F &&f) const {
+        each(f, [] {});
     }
 };
 
@@ -246,6 +295,10 @@ __device__ __forceinline__ void fused_pass(const Points<G, R> &pts, const float 
             acc[TRI + k].add_prod(jac[k], r);
         }
         acc[TRI + P].add_prod(r, r);
+    }, [&] {
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            acc[t].flush();
     });
     lin_ok = true;
 #pragma unroll
@@ -280,6 +333,10 @@ __device__ __forceinline__ void information_pass(const Points<G, R> &pts, const 
             for (int l = 0; l <= k; ++l)
                 acc[tri(k, l)].add_prod(jac[k], jac[l]);
         }
+    }, [&] {
+#pragma unroll
+        for (int t = 0; t < TRI; ++t)
+            acc[t].flush();
     });
     if constexpr (G > 1)
         bad = group_or<G>(bad);
@@ -443,11 +500,12 @@ __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R> 
 
 // The LM driver for one fit, run redundantly by every lane of its group.
 template <int MODEL, int G, int R, bool COMP, typename S>
-__device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl) {
+__device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl,
+                                          const float *xgrid) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     const GridInfo grid = *a.grid;
     Points<G, R> pts;
-    pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
+    pts.load(xgrid, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
 
     float init[P], p[P];
     bool bad = !a.options_ok || !grid.x_finite;
@@ -619,14 +677,28 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
         covariance<MODEL, G, R, S>(a, pts, fit, gl, p, status, chi_out);
 }
 
+// Occupancy floor per instantiation: three waves per SIMD (≤ 168 VGPRs) where
+// the state fits with at most a few spilled registers; otherwise unconstrained.
+constexpr int lm_min_waves(int model, int g, int r) {
+    return (model != TWO_LORENTZIAN && g == 64 && r > 0 && r <= 16) ? 3 : 1;
+}
+
 template <int MODEL, int G, int R, bool COMP, bool MIXED>
-__global__ void __launch_bounds__(256) lm_kernel(FitArgs a) {
+__global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, R)) lm_kernel(FitArgs a) {
     constexpr int per_block = 256 / G;
-    const uint32_t fit = blockIdx.x * per_block + threadIdx.x / G;
-    if (fit >= a.n)
-        return;
     using S = typename std::conditional<MIXED, double, float>::type;
-    solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x % G));
+    const uint32_t fit = blockIdx.x * per_block + threadIdx.x / G;
+    if constexpr (R > 0) {
+        __shared__ float xs[G * R];
+        for (int j = threadIdx.x; j < G * R; j += 256)
+            xs[j] = j < int(a.m) ? a.x[j] : 0.0f;
+        __syncthreads();
+        if (fit < a.n)
+            solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x % G), xs);
+    } else {
+        if (fit < a.n)
+            solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x % G), a.x);
+    }
 }
 
 }  // namespace bfhip

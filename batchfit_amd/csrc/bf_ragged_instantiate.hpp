@@ -1,0 +1,56 @@
+// bf_ragged_instantiate.hpp — included by bf_ragged_<model>.hip with BF_MODEL
+// set; defines that model's ragged launchers (four mapping classes × the
+// plain FP32 and compensated-mixed numerics).
+#pragma once
+
+#include "bf_ragged.hpp"
+
+#ifndef BF_MODEL
+#error "define BF_MODEL before including bf_ragged_instantiate.hpp"
+#endif
+
+namespace bfhip {
+
+namespace {
+template <int MODEL, int CLS, bool MIXED>
+hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream) {
+    constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    hipLaunchKernelGGL((lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>), dim3(blocks), dim3(256), 0,
+                       stream, args);
+    return hipGetLastError();
+}
+template <int MODEL, bool MIXED>
+hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hipStream_t stream) {
+    switch (cls) {
+    case 0:
+        return ragged_go<MODEL, 0, MIXED>(args, blocks, stream);
+    case 1:
+        return ragged_go<MODEL, 1, MIXED>(args, blocks, stream);
+    case 2:
+        return ragged_go<MODEL, 2, MIXED>(args, blocks, stream);
+    case 3:
+        return ragged_go<MODEL, 3, MIXED>(args, blocks, stream);
+    }
+    return hipErrorInvalidValue;
+}
+}  // namespace
+
+template <> hipError_t launch_ragged_lm<BF_MODEL>(int cls, bool mixed, const RaggedArgs &args,
+                                                  uint32_t blocks, hipStream_t stream) {
+    return mixed ? ragged_by_class<BF_MODEL, true>(cls, args, blocks, stream)
+                 : ragged_by_class<BF_MODEL, false>(cls, args, blocks, stream);
+}
+
+template <>
+hipError_t launch_ragged_guess<BF_MODEL>(const bf_fit_desc *fits, const uint32_t *order,
+                                         uint32_t count, const bf_grid_desc *grids,
+                                         const GridInfo *grid_info, const float *x, const float *y,
+                                         float *guess, uint32_t m_pad, uint32_t sort_n,
+                                         hipStream_t stream) {
+    const size_t lds = size_t(m_pad + sort_n) * sizeof(float);
+    hipLaunchKernelGGL((guess_ragged_kernel<BF_MODEL>), dim3(count), dim3(64), lds, stream, fits,
+                       order, count, grids, grid_info, x, y, guess, m_pad, sort_n);
+    return hipGetLastError();
+}
+
+}  // namespace bfhip

@@ -6,9 +6,7 @@
 // and the binding's host preparation (python/bindings.cpp:60-121): the guess
 // runs on the device instead of a 2-worker CPU pool, and y stays fit-major in
 // HBM for the group mappings instead of a host 16×16 transpose.
-#include "bf_launch.hpp"
-
-#include "../../include/batchfit_hip.h"
+#include "bf_host.hpp"
 
 #include <atomic>
 #include <cmath>
@@ -30,58 +28,8 @@ namespace {
 // ---------------------------------------------------------------------------
 // Device kernels owned by the runtime.
 
-// Shared-grid facts (cpu/solver.hpp:58-69, batch.hpp:236-240,
-// initial_guess.hpp:131-153) in one 256-thread block. Minima are exact, so the
-// parallel order gives the reference's sequential values.
 __global__ void __launch_bounds__(256) grid_kernel(const float *x, uint32_t m, GridInfo *out) {
-    __shared__ float s_abs[256], s_gap[256];
-    __shared__ int s_flags[256];
-    const int t = threadIdx.x;
-    const float x0 = x[0];
-    float amin = __uint_as_float(0x7f800000u), gmin = 3.40282346638528859812e+38f;
-    int fl = 0;  // 1 non-finite, 2 not rising, 4 not falling, 8 varied, 16 not strictly rising
-    for (uint32_t j = t; j < m; j += 256) {
-        const float xj = x[j];
-        fl |= finite_f(xj) ? 0 : 1;
-        fl |= (xj != x0) ? 8 : 0;
-        if (j > 0) {
-            const float prev = x[j - 1];
-            const float d = xj - prev;
-            fl |= (d >= 0.0f) ? 0 : 2;
-            fl |= (d <= 0.0f) ? 0 : 4;
-            if (d != 0.0f)
-                amin = fminf(amin, fabsf(d));
-            fl |= (xj <= prev) ? 16 : 0;
-            gmin = fminf(gmin, d);
-        }
-    }
-    s_abs[t] = amin;
-    s_gap[t] = gmin;
-    s_flags[t] = fl;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) {
-            s_abs[t] = fminf(s_abs[t], s_abs[t + o]);
-            s_gap[t] = fminf(s_gap[t], s_gap[t + o]);
-            s_flags[t] |= s_flags[t + o];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        const int f = s_flags[0];
-        const bool monotone = !(f & 2) || !(f & 4);
-        GridInfo g;
-        g.min_spacing = monotone && finite_f(s_abs[0]) ? s_abs[0] : 0.0f;
-        g.varied_x = (f & 8) ? 1 : 0;
-        g.x_finite = (f & 1) ? 0 : 1;
-        g.guess_grid_ok = !(f & 1) && !(f & 16) && finite_f(x[m - 1] - x[0]);
-        g.guess_spacing = s_gap[0];
-        const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-        g.count = int(count);
-        g.left_x = std_midpoint(x[(count - 1) / 2], x[count / 2]);
-        g.right_x = std_midpoint(x[m - count + (count - 1) / 2], x[m - count + count / 2]);
-        *out = g;
-    }
+    grid_block(x, m, out);
 }
 
 // Fit-major [n,m] → point-major [m,n] (the layout of the reference GPU
@@ -107,63 +55,6 @@ __global__ void __launch_bounds__(256) transpose_kernel(const float *in, float *
 // ---------------------------------------------------------------------------
 // Host side.
 
-constexpr int kParams[4] = {3, 3, 4, 8};
-
-int fail(char *err, size_t errlen, int code, const std::string &message) {
-    if (err && errlen) {
-        std::snprintf(err, errlen, "%s", message.c_str());
-    }
-    return code;
-}
-
-#define BF_TRY(expr, what)                                                                     \
-    do {                                                                                       \
-        const hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(err, errlen, BF_RUNTIME_ERROR,                                         \
-                        std::string(what) + " failed: " + hipGetErrorString(e_));             \
-    } while (0)
-
-struct Runtime {
-    int status = BF_UNAVAILABLE;
-    std::string why = "no HIP device";
-    hipStream_t stream = nullptr;
-    std::mutex host_calls;  // serialises the synchronous host-array entry (bindings.cpp:13)
-
-    Runtime() {
-        int count = 0;
-        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
-            why = "no HIP device is visible";
-            return;
-        }
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
-            why = "hipGetDeviceProperties failed";
-            return;
-        }
-        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-            why = std::string("device 0 is ") + prop.gcnArchName + ", this library targets gfx950";
-            return;
-        }
-        if (hipSetDevice(0) != hipSuccess ||
-            hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-            why = "could not create a HIP stream";
-            return;
-        }
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
-            uint64_t keep = std::numeric_limits<uint64_t>::max();
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        }
-        status = BF_OK;
-        why.clear();
-    }
-    static Runtime &get() {
-        static Runtime runtime;
-        return runtime;
-    }
-};
-
 std::atomic<bool> g_timing{std::getenv("BF_HIP_TIMING") != nullptr};
 
 struct Timing {
@@ -183,14 +74,7 @@ hipError_t record(int i, hipStream_t stream) {
 }
 
 // Tuned default mapping by (model, M) (replaces metal::dispatch_variant,
-// include/batchfit/metal/batch.hpp:21-50). Every row keeps y in registers.
-int pow2_ceil(uint32_t v) {
-    int r = 1;
-    while (uint32_t(r) < v)
-        r <<= 1;
-    return r;
-}
-
+// include/batchfit/metal/batch.hpp:21-50).
 int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *launch, Plan &plan,
                 char *err, size_t errlen) {
     plan.comp = (numerics & BF_COMPENSATED) != 0;
@@ -201,16 +85,17 @@ int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *
         plan.R = 0;
         return BF_OK;
     }
-    int g = 64;
+    int g = tuned_group(int(model), m);
     if (mapping == BF_MAP_GROUP && launch->group_lanes)
         g = int(launch->group_lanes);
     else if (mapping != BF_MAP_GROUP && mapping != BF_MAP_AUTO)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown mapping");
-    (void)model;
+    if (mapping == BF_MAP_AUTO && g != 64 && !plan_instantiated(g, 2, plan.comp, plan.mixed))
+        g = 64;  // 16/32-lane groups exist for numerics 0 and 5 only
     if (g != 16 && g != 32 && g != 64)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "group_lanes must be 16, 32 or 64");
     const uint32_t per_lane = (m + uint32_t(g) - 1) / uint32_t(g);
-    int r = pow2_ceil(per_lane);
+    int r = pow2_ceil(per_lane < 2 ? 2 : per_lane);
     if (r > kMaxTile) {
         if (g != 64)
             return fail(err, errlen, BF_INVALID_ARGUMENT,
@@ -341,7 +226,7 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     const float *init = p0;
     if (!init) {
         const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-        const uint32_t sort_n = uint32_t(pow2_ceil(2 * count));
+        const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
         const size_t lds = size_t(m + sort_n) * sizeof(float);
         if (lds > 65536)
             return fail(err, errlen, BF_INVALID_ARGUMENT,
@@ -549,7 +434,7 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     hipStream_t s = rt.stream;
     const size_t P = size_t(kParams[model]);
     const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-    const uint32_t sort_n = uint32_t(pow2_ceil(2 * count));
+    const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
     const size_t lds = size_t(m + sort_n) * sizeof(float);
     if (lds > 65536)
         return fail(err, errlen, BF_INVALID_ARGUMENT,

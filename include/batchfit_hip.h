@@ -77,6 +77,24 @@ typedef struct bf_launch {
     void *stream;         /* hipStream_t; NULL = the library's stream */
 } bf_launch;
 
+/* One fit of a ragged batch (mixed model, M and options in one launch). */
+typedef struct bf_fit_desc {
+    uint64_t y_offset; /* index of the fit's first observation in y */
+    uint32_t points;   /* M of this fit; must equal its grid's points */
+    uint32_t grid;     /* row of the grid table: the fit's x */
+    uint32_t model;    /* bf_model */
+    uint32_t config;   /* row of the config table (bounds, tolerances, numerics) */
+    uint32_t p0_row;   /* row of p0 (rows of 8 floats), or 0xffffffff for the device guess */
+    uint32_t out_row;  /* row of the outputs: params/stderr rows of 8, cov rows of 64 */
+} bf_fit_desc;
+
+/* One shared grid of a ragged batch: x[x_offset .. x_offset+points). */
+typedef struct bf_grid_desc {
+    uint64_t x_offset;
+    uint32_t points;
+    uint32_t reserved;
+} bf_grid_desc;
+
 /* Library and device facts. */
 int bf_hip_abi_version(void);
 /* 1 if a gfx950 device is usable (bindings.cpp:208-211 metal_available). */
@@ -103,6 +121,31 @@ int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *lau
                       const float *d_x, const float *d_y, const float *d_p0, float *d_params,
                       int32_t *d_status, uint32_t *d_iterations, float *d_chi2, float *d_cov,
                       float *d_stderr, int32_t *d_ustatus, char *err, size_t errlen);
+
+/* Ragged batches: fits of any model and M, each with its own config row, in
+ * one hipGraph launch (persistent kernels fed by device work queues).
+ * bf_hip_ragged_create takes HOST descriptor tables, validates them, buckets
+ * the fits by (model, mapping class), orders every class longest-first and
+ * uploads everything; numerics must be 0 (FP32) or 5 (compensated mixed).
+ * bf_hip_ragged_run enqueues the plan on its stream over DEVICE data: x
+ * (grids packed), y (observations packed), p0 rows of 8 (or NULL when no
+ * descriptor names a p0 row) and outputs with rows of 8 (params, stderr), 64
+ * (cov) or 1 (status, iterations, chi2, ustatus); cov/stderr/ustatus only when
+ * the plan was created with uncertainties. The captured graph is reused while
+ * the pointers stay the same. No counterpart in the reference (north-star
+ * batching); each fit is solved as a uniform batch of one would be. */
+typedef struct bf_ragged_plan_s *bf_ragged_plan;
+int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc *fit_table,
+                         uint32_t grids, const bf_grid_desc *grid_table, uint32_t configs,
+                         const bf_config *config_table, uint32_t numerics, int uncertainties,
+                         void *stream, char *err, size_t errlen);
+int bf_hip_ragged_run(bf_ragged_plan plan, const float *d_x, const float *d_y, const float *d_p0,
+                      float *d_params, int32_t *d_status, uint32_t *d_iterations, float *d_chi2,
+                      float *d_cov, float *d_stderr, int32_t *d_ustatus, char *err,
+                      size_t errlen);
+/* Device timing of the last run of the plan (ms): [0] prologue + guess, [1] LM. */
+int bf_hip_ragged_timing(bf_ragged_plan plan, double *times, int count);
+int bf_hip_ragged_destroy(bf_ragged_plan plan);
 
 /* Data-only initial guess alone, host arrays (cpu/initial_guess.hpp:120-249):
  * params[N*P], baseline[N], gstatus[N] (GuessStatus codes 0..3). */

@@ -23,6 +23,13 @@ NUMERICS = [0, 5]  # FP32 as the reference GPU kernels; compensated FP32 + FP64 
 P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
 
+def mappings_for(m):
+    """Mappings whose register tile (at most 32 points per lane) holds M."""
+    return [mp for mp in MAPPINGS
+            if mp in ("lane", "group64") or (mp == "group32" and m <= 1024)
+            or (mp == "group16" and m <= 512)]
+
+
 def assert_fp32_gate(got, ref, label):
     bad = np.nonzero(got.status != ref.status)[0]
     assert bad.size == 0, f"{label}: status differs at rows {bad[:10]} " \
@@ -123,7 +130,7 @@ def test_fit_matches_fp32_oracle(hip, oracle_port, model, m, n):
     x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
     x, y = x.astype(np.float32), y.astype(np.float32)
     ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
-    for mapping in MAPPINGS:
+    for mapping in mappings_for(m):
         for numerics in NUMERICS:
             got = hip.fit(x, y, model, p0, bounds, "hip", mapping=mapping, numerics=numerics)
             assert got.params.dtype == np.float32 and got.status.dtype == np.int32
@@ -171,6 +178,8 @@ def test_metric_workload_vs_fp64_oracle(hip, oracle_port):
 @pytest.mark.parametrize("mapping", MAPPINGS)
 @pytest.mark.parametrize("numerics", NUMERICS)
 def test_failure_isolation(hip, golden, mapping, numerics):
+    """Includes a 16-lane request for M=1024, which has no register tile and
+    must be an argument error rather than a launch."""
     """tests/metal_lm_checks.hpp:59-128 and cooperative_checks.hpp:203-257."""
     g = golden["failures"]
     x, y, p0 = datasets.isolation_fixture()
@@ -194,6 +203,10 @@ def test_failure_isolation(hip, golden, mapping, numerics):
     assert r.status[0] == 2 and r.status[1] == 3
     empty = hip.fit(np.linspace(0, 1, 32), np.empty((0, 32)), "gaussian", None, None, "hip", **kw)
     assert empty.params.shape == (0, 3) and empty.status.shape == (0,)
+    if mapping == "group16":
+        wide = np.linspace(-1, 1, 1024)
+        with pytest.raises(ValueError, match="register tile"):
+            hip.fit(wide, np.ones((2, 1024)), "gaussian", [1, 0, 1], None, "hip", **kw)
 
 
 @pytest.mark.parametrize("model", list(P))
@@ -298,7 +311,6 @@ def test_point_count_edges(hip, oracle_port, m):
     x, y, p0, bounds, _ = datasets.comparison_set("pseudo_voigt", 48, m)
     x, y = x.astype(np.float32), y.astype(np.float32)
     ref = oracle_port.fit(x, y, "pseudo_voigt", p0, bounds, threads=8)
-    maps = MAPPINGS if m <= 1024 else ["lane", "group64"]
-    for mapping in maps:
+    for mapping in mappings_for(m):
         assert_fp32_gate(hip.fit(x, y, "pseudo_voigt", p0, bounds, "hip", mapping=mapping), ref,
                          f"M={m} {mapping}")

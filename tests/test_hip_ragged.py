@@ -1,0 +1,133 @@
+"""GPU tests of ragged batches (bf_hip_ragged_*): fits of mixed model, M and
+bounds in one captured launch.
+
+* Same alone and in the mix: every fit's bytes equal those of the same fit in
+  a plan of its own batch (the mapping depends on (model, M) only).
+* Parity: each batch within the reference's FP32 GPU gate of the FP32 oracle
+  (params ≤ 2e-4 + 2e-4·|cpu|, χ² ≤ 1e-6 + 2e-5·|cpu|, equal status).
+* Graph replays: the captured graph replayed several times, with eager work
+  between replays, gives the eager result every time.
+"""
+import numpy as np
+import pytest
+
+import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                          np.ascontiguousarray(b).view(np.uint8))
+
+
+def mixed_batches():
+    out = []
+    for model in datasets.MODELS:
+        for m, n in ((64, 37), (129, 40), (300, 21), (1024, 19), (1500, 9), (2500, 5)):
+            x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
+            out.append((x.astype(np.float32), y.astype(np.float32), model,
+                        None if m in (129, 1024) else p0, bounds))
+    return out
+
+
+def test_ragged_matches_oracle_and_is_batch_independent(hip, oracle_port):
+    from batchfit_amd.ragged import fit_ragged
+    batches = mixed_batches()
+    together = fit_ragged(batches)
+    for (x, y, model, p0, bounds), got in zip(batches, together):
+        ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
+        label = f"{model} M={x.size}"
+        assert np.array_equal(got.status, ref.status), label
+        ok = ref.status == 0
+        assert np.all(np.abs(got.params[ok] - ref.params[ok])
+                      <= 2e-4 + 2e-4 * np.abs(ref.params[ok])), label
+        assert np.all(np.abs(got.chi2[ok] - ref.chi2[ok]) <= 1e-6 + 2e-5 * np.abs(ref.chi2[ok]))
+        alone = fit_ragged([(x, y, model, p0, bounds)])[0]
+        for a, b in zip(alone, got):
+            assert same(a, b), f"{label}: result depends on the rest of the batch"
+        one = fit_ragged([(x, y[:1], model, p0, bounds)])[0]
+        for a, b in zip(one, got):
+            assert same(a, b[:1]), label
+
+
+def test_ragged_equals_uniform_path(hip):
+    """The uniform entry uses the same tuned (model, M) table: same bytes."""
+    from batchfit_amd.ragged import fit_ragged
+    for model in ("lorentzian", "pseudo_voigt"):
+        for m in (100, 700, 1800):
+            x, y, p0, bounds, _ = datasets.comparison_set(model, 33, m)
+            x, y = x.astype(np.float32), y.astype(np.float32)
+            a = hip.fit(x, y, model, p0, bounds, "hip")
+            b = fit_ragged([(x, y, model, p0, bounds)])[0]
+            for u, v in zip(a, b):
+                assert same(u, v), (model, m)
+
+
+def test_ragged_uncertainties_and_failures(hip, oracle_port):
+    from batchfit_amd.ragged import fit_ragged
+    x, y, p0, bounds, _ = datasets.comparison_set("gaussian", 20, 129)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    y[3, 7] = np.nan
+    y[4] = 1.0
+    xi, yi, pi = datasets.isolation_fixture()
+    res = fit_ragged([(x, y, "gaussian", p0, bounds), (xi, yi, "lorentzian", pi, None)],
+                     uncertainties=True)
+    g = res[0]
+    assert g.status[3] == 3 and g.status[4] == 4
+    assert g.covariance.shape == (20, 3, 3)
+    for i in range(20):
+        cov, se, us = oracle_port.uncertainty("gaussian", x, g.params[i], int(g.status[i]),
+                                              float(g.chi2[i]), bounds, dtype=np.float32)
+        assert g.uncertainty_status[i] == us
+        if us == 0:
+            np.testing.assert_allclose(g.covariance[i], cov, rtol=2e-4, atol=1e-9)
+    iso = res[1]
+    assert list(iso.status) == [2, 0, 3] and list(iso.iterations) == [1, 0, 0]
+
+
+def test_graph_replays_match_eager(hip):
+    from batchfit_amd import ragged
+    batches = mixed_batches()[:8]
+    first = ragged.fit_ragged(batches)
+    # Build one plan and replay its graph several times with eager work between.
+    import batchfit_amd._native as native
+    xs, ys, descs, grids, configs = [], [], [], [], []
+    x_off = y_off = row = 0
+    for b, (x, y, model, p0, bounds) in enumerate(batches):
+        n, m = y.shape
+        cfg = native.default_config()
+        cfg.numerics = 5
+        for k in range(len(bounds[0])):
+            cfg.lower[k], cfg.upper[k] = float(bounds[0][k]), float(bounds[1][k])
+        configs.append(cfg)
+        grids.append((x_off, m, 0))
+        d = np.zeros(n, ragged.FIT_DESC)
+        d["y_offset"] = y_off + np.arange(n) * m
+        d["points"], d["grid"], d["model"], d["config"] = m, b, native.MODELS[model], b
+        d["p0_row"], d["out_row"] = ragged.NO_P0, row + np.arange(n)
+        descs.append(d)
+        xs.append(x)
+        ys.append(y.reshape(-1))
+        x_off, y_off, row = x_off + m, y_off + n * m, row + n
+    plan = ragged.RaggedPlan(np.concatenate(descs), np.array(grids, ragged.GRID_DESC), configs)
+    dx, dy = ragged.DeviceArray.of(np.concatenate(xs)), ragged.DeviceArray.of(np.concatenate(ys))
+    out = ragged.allocate_outputs(row, False)
+    expected = np.concatenate([r.params for r in first], axis=None)
+    for replay in range(4):
+        plan.run(dx, dy, None, out)
+        host = ragged.download_outputs(out, row)
+        got = np.concatenate([host["params"][s:s + n, :p].reshape(-1) for s, n, p in
+                              _shapes(batches)])
+        assert same(got, expected), f"replay {replay}"
+        eager = hip.fit(batches[0][0], batches[0][1], batches[0][2], None, batches[0][4], "hip")
+        assert eager.params.shape[0] == batches[0][1].shape[0]
+    plan.close()
+
+
+def _shapes(batches):
+    P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
+    row = 0
+    for x, y, model, p0, bounds in batches:
+        yield row, y.shape[0], P[model]
+        row += y.shape[0]
