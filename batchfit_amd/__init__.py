@@ -82,35 +82,12 @@ def _real_array(value, name):
     return array
 
 
-_MAPPINGS = {"auto": (0, 0), "lane": (1, 0), "group16": (2, 16), "group32": (2, 32),
-             "group64": (2, 64), "wave": (2, 64)}
-
-
-def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncertainties=False,
-        numerics=None, mapping="auto", max_iterations=100):
-    """Fit shared x[M] and y[N,M]; return params[N,P], status/iterations/chi2[N].
-
-    A single y[M] is accepted as N=1. ``device`` is "auto" or "hip"; both run
-    on the GPU in float32 and never fall back to a CPU path. ``p0=None`` runs
-    the data-only initial guess on the device. ``uncertainties=True`` adds
-    covariance[N,P,P], stderr[N,P] and uncertainty_status[N].
-
-    Keyword-only extensions of the HIP device: ``numerics`` (bit0 compensated
-    partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
-    ``mapping`` ("auto", "lane", "group16", "group32", "group64") and
-    ``max_iterations`` (default 100, cpu/solver.hpp:27).
-    """
+def _validate_batch(x, y, model, p0, bounds):
+    """Argument checks of the reference's ``fit`` (python/batchfit/__init__.py:63-112,
+    python/bindings.cpp:153-161); returns float32 x[M], y[N,M], p0[N,P] or None
+    and the broadcast bounds lower[P], upper[P]."""
     if model not in _PARAMETERS:
         raise ValueError(f"unknown model {model!r}; choose one of {tuple(_PARAMETERS)}")
-    if device == "cpu":
-        raise ValueError("device 'cpu' is the reference CPU path, which this package does not "
-                         "include; use device='hip'")
-    if device not in ("auto", "hip"):
-        raise ValueError("device must be 'auto' or 'hip'")
-    if not isinstance(uncertainties, (bool, np.bool_)):
-        raise TypeError("uncertainties must be a boolean")
-    if mapping not in _MAPPINGS:
-        raise ValueError(f"mapping must be one of {tuple(_MAPPINGS)}")
     x, y = _real_array(x, "x"), _real_array(y, "y")
     if x.ndim != 1 or not x.size:
         raise ValueError("x must have nonempty shape [M]")
@@ -148,16 +125,57 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
     if (np.any(np.isnan(lower)) or np.any(np.isnan(upper)) or np.any(lower > upper)
             or np.any(lower == np.inf) or np.any(upper == -np.inf)):
         raise ValueError("invalid parameter bounds")
+    return x, y, p0, lower, upper
+
+
+def _make_config(n, m, lower, upper, max_iterations, numerics):
+    cfg = _native.default_config()
+    cfg.fits, cfg.points = int(n), int(m)
+    cfg.max_iterations = int(max_iterations)
+    cfg.numerics = int(numerics)
+    for k in range(len(lower)):
+        cfg.lower[k], cfg.upper[k] = float(lower[k]), float(upper[k])
+    return cfg
+
+
+_MAPPINGS = {"auto": (0, 0), "lane": (1, 0), "group16": (2, 16), "group32": (2, 32),
+             "group64": (2, 64), "wave": (2, 64)}
+
+
+def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncertainties=False,
+        numerics=None, mapping="auto", max_iterations=100):
+    """Fit shared x[M] and y[N,M]; return params[N,P], status/iterations/chi2[N].
+
+    A single y[M] is accepted as N=1. ``device`` is "auto" or "hip"; both run
+    on the GPU in float32 and never fall back to a CPU path. ``p0=None`` runs
+    the data-only initial guess on the device. ``uncertainties=True`` adds
+    covariance[N,P,P], stderr[N,P] and uncertainty_status[N].
+
+    Keyword-only extensions of the HIP device: ``numerics`` (bit0 compensated
+    partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
+    ``mapping`` ("auto", "lane", "group16", "group32", "group64") and
+    ``max_iterations`` (default 100, cpu/solver.hpp:27).
+    """
+    if model not in _PARAMETERS:
+        raise ValueError(f"unknown model {model!r}; choose one of {tuple(_PARAMETERS)}")
+    if device == "cpu":
+        raise ValueError("device 'cpu' is the reference CPU path, which this package does not "
+                         "include; use device='hip'")
+    if device not in ("auto", "hip"):
+        raise ValueError("device must be 'auto' or 'hip'")
+    if not isinstance(uncertainties, (bool, np.bool_)):
+        raise TypeError("uncertainties must be a boolean")
+    if mapping not in _MAPPINGS:
+        raise ValueError(f"mapping must be one of {tuple(_MAPPINGS)}")
+    x, y, p0, lower, upper = _validate_batch(x, y, model, p0, bounds)
+    dtype = np.dtype(np.float32)
+    n, p = y.shape[0], _PARAMETERS[model]
 
     lib = _native.library()
     if not lib.bf_hip_available():
         raise RuntimeError("HIP fitting is unavailable in this build or on this device")
-    cfg = _native.default_config()
-    cfg.fits, cfg.points = n, x.size
-    cfg.max_iterations = int(max_iterations)
-    cfg.numerics = NUMERICS_DEFAULT if numerics is None else int(numerics)
-    for k in range(p):
-        cfg.lower[k], cfg.upper[k] = float(lower[k]), float(upper[k])
+    cfg = _make_config(n, x.size, lower, upper, max_iterations,
+                       NUMERICS_DEFAULT if numerics is None else numerics)
     launch = _native.Launch()
     launch.mapping, launch.group_lanes = _MAPPINGS[mapping]
 

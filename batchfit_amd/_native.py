@@ -15,7 +15,7 @@ LIBRARY = os.path.join(HERE, "libbatchfit_hip.so")
 MODELS = {"gaussian": 0, "lorentzian": 1, "pseudo_voigt": 2, "two_lorentzian": 3}
 PARAMETERS = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
-BF_OK, BF_INVALID_ARGUMENT, BF_RUNTIME_ERROR, BF_UNAVAILABLE = 0, 1, 2, 3
+BF_OK, BF_INVALID_ARGUMENT, BF_RUNTIME_ERROR, BF_UNAVAILABLE, BF_TIMEOUT = 0, 1, 2, 3, 4
 COMPENSATED, SCALED_CHOLESKY, MIXED_FP64 = 1, 2, 4
 MAP_AUTO, MAP_LANE, MAP_GROUP = 0, 1, 2
 
@@ -25,6 +25,8 @@ EXPORTS = (
     "bf_hip_fit_device", "bf_hip_guess", "bf_hip_model_eval", "bf_hip_set_timing",
     "bf_hip_last_timing", "bf_hip_malloc", "bf_hip_free", "bf_hip_memcpy", "bf_hip_synchronize",
     "bf_hip_ragged_create", "bf_hip_ragged_run", "bf_hip_ragged_timing", "bf_hip_ragged_destroy",
+    "bf_hip_synthesize", "bf_serve_create", "bf_serve_submit", "bf_serve_wait", "bf_serve_stats",
+    "bf_serve_destroy",
 )
 
 

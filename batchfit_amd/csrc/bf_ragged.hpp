@@ -14,6 +14,9 @@
 
 #include "../../include/batchfit_hip.h"
 
+#include <string>
+#include <vector>
+
 namespace bfhip {
 
 struct RaggedArgs {
@@ -83,29 +86,33 @@ __device__ __forceinline__ FitArgs ragged_fit_args(const RaggedArgs &r, const bf
     return a;
 }
 
-// Broadcast lane 0 of each G-lane group to the group.
-template <int G> __device__ __forceinline__ uint32_t group_first(uint32_t v, int gl) {
-    if constexpr (G == 64)
-        return __builtin_amdgcn_readfirstlane(v);
-    else
-        return uint32_t(group_max_i<G>(gl == 0 ? int(v) : -1));
-}
-
+// Persistent LM kernel of one class. The work queue is taken per wave: lane 0
+// of the wave adds 1 to the class counter and the wave's 64/G groups solve
+// tasks base·(64/G) … base·(64/G)+64/G-1. The queue read is branch-free (every
+// lane issues the add, only lane 0 adds 1) and the loop exit is a scalar
+// branch on a wave-uniform value, so no lane of a group can reach a group
+// reduction on a path that skipped the queue read: with a per-group exit, the
+// optimiser may thread the lanes with gl != 0 straight back to the group
+// broadcast, which then reads lane 0 while it is inactive.
 template <int MODEL, int G, int R, bool COMP, bool MIXED>
 __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, R)) lm_ragged_kernel(RaggedArgs r) {
     using S = typename std::conditional<MIXED, double, float>::type;
     constexpr int P = Params<MODEL>::P;
-    const int gl = int(threadIdx.x % G);
+    constexpr uint32_t per_wave = 64 / G;
+    const int lane = int(threadIdx.x % 64);
+    const int gl = lane % G;
+    const uint32_t g = uint32_t(lane / G);
     for (;;) {
-        uint32_t task = 0;
-        if (gl == 0)
-            task = atomicAdd(r.cursor, 1u);
-        task = group_first<G>(task, gl);
-        if (task >= r.count)
+        const uint32_t ticket = atomicAdd(r.cursor, lane == 0 ? 1u : 0u);
+        const uint32_t base = __builtin_amdgcn_readfirstlane(ticket) * per_wave;
+        if (base >= r.count)
             break;
-        const bf_fit_desc d = r.fits[r.order[task]];
-        const FitArgs a = ragged_fit_args(r, d, MODEL, P);
-        solve_fit<MODEL, G, R, COMP, S>(a, 0, gl, a.x);
+        const uint32_t task = base + g;
+        if (task < r.count) {
+            const bf_fit_desc d = r.fits[r.order[task]];
+            const FitArgs a = ragged_fit_args(r, d, MODEL, P);
+            solve_fit<MODEL, G, R, COMP, S>(a, 0, gl, a.x);
+        }
     }
 }
 
@@ -153,6 +160,53 @@ inline int ragged_class_of(uint32_t m) {
     return 3;
 }
 
+// One class of a ragged batch: a slice of the work order.
+struct RaggedRun {
+    uint32_t model;
+    int cls;
+    uint32_t offset, count;  // slice of the order array
+    uint32_t guesses;        // fits in the slice that start from the device guess
+    uint32_t max_points;
+};
+
+// Host view of a ragged batch: validated descriptors bucketed by (model,
+// class), each bucket longest-first.
+struct RaggedLayout {
+    std::vector<uint32_t> order;
+    std::vector<RaggedRun> runs;
+    bool any_guess = false, any_p0 = false;
+    uint32_t out_rows = 0;
+};
+
+// Device tables of a ragged batch.
+struct RaggedDevice {
+    const bf_fit_desc *fits;
+    const bf_grid_desc *grids;
+    const bf_config *configs;
+    GridInfo *info;
+    const uint32_t *order;
+    uint32_t *cursors;  // ≥ runs.size() counters
+    float *guess;       // out_rows × 8 when any fit starts from the guess
+    uint32_t grid_count;
+};
+
+// Validate and bucket; returns an empty string or the reason for rejection.
+std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
+                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out);
+
+// Enqueue a whole ragged batch on `stream` (no synchronisation, no capture):
+// queue reset, grid table, guesses, persistent LM kernels. `events` (3 or
+// null) are recorded before the prologue, before the LM and at the end.
+hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, bool mixed,
+                          const float *x, const float *y, const float *p0, float *params,
+                          int32_t *status, uint32_t *iterations, float *chi2, float *cov,
+                          float *stderr_, int32_t *ustatus, hipStream_t stream,
+                          const hipEvent_t *events);
+
+// Load the kernels a layout launches (outside any capture).
+hipError_t ragged_prepare(const RaggedLayout &layout, bool mixed);
+
+template <int MODEL> hipError_t prepare_ragged(int cls, bool mixed, bool guess);
 template <int MODEL>
 hipError_t launch_ragged_lm(int cls, bool mixed, const RaggedArgs &args, uint32_t blocks,
                             hipStream_t stream);

@@ -33,7 +33,33 @@ hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hip
     }
     return hipErrorInvalidValue;
 }
+template <int MODEL, int CLS, bool MIXED> const void *ragged_symbol() {
+    constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    return reinterpret_cast<const void *>(&lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>);
+}
 }  // namespace
+
+// Loads the code object of a class's kernels (hipFuncGetAttributes) so that
+// the first use inside a stream capture does not have to.
+template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess) {
+    const void *sym = nullptr;
+    switch (cls * 2 + (mixed ? 1 : 0)) {
+    case 0: sym = ragged_symbol<BF_MODEL, 0, false>(); break;
+    case 1: sym = ragged_symbol<BF_MODEL, 0, true>(); break;
+    case 2: sym = ragged_symbol<BF_MODEL, 1, false>(); break;
+    case 3: sym = ragged_symbol<BF_MODEL, 1, true>(); break;
+    case 4: sym = ragged_symbol<BF_MODEL, 2, false>(); break;
+    case 5: sym = ragged_symbol<BF_MODEL, 2, true>(); break;
+    case 6: sym = ragged_symbol<BF_MODEL, 3, false>(); break;
+    case 7: sym = ragged_symbol<BF_MODEL, 3, true>(); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, sym);
+    if (e == hipSuccess && guess)
+        e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&guess_ragged_kernel<BF_MODEL>));
+    return e;
+}
 
 template <> hipError_t launch_ragged_lm<BF_MODEL>(int cls, bool mixed, const RaggedArgs &args,
                                                   uint32_t blocks, hipStream_t stream) {

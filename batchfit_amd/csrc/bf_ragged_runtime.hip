@@ -11,13 +11,6 @@
 
 namespace bfhip {
 
-template <int MODEL> hipError_t launch_ragged_lm(int, bool, const RaggedArgs &, uint32_t,
-                                                 hipStream_t);
-template <int MODEL>
-hipError_t launch_ragged_guess(const bf_fit_desc *, const uint32_t *, uint32_t,
-                               const bf_grid_desc *, const GridInfo *, const float *,
-                               const float *, float *, uint32_t, uint32_t, hipStream_t);
-
 namespace {
 
 __global__ void __launch_bounds__(64) reset_kernel(uint32_t *cursors, uint32_t n) {
@@ -57,13 +50,129 @@ hipError_t ragged_guess(uint32_t model, const bf_fit_desc *fits, const uint32_t 
 
 }  // namespace
 
-struct RaggedClassRun {
-    uint32_t model;
-    int cls;
-    uint32_t offset, count;  // slice of the order array
-    uint32_t guesses;        // fits in the slice that start from the device guess
-    uint32_t max_points;
-};
+std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
+                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out) {
+    out = RaggedLayout{};
+    if (fits && (!fit_table || !grid_table || !grids || !configs))
+        return "descriptor tables are required";
+    for (uint32_t g = 0; g < grids; ++g)
+        if (grid_table[g].points == 0)
+            return "every grid needs M>0";
+    std::vector<std::vector<uint32_t>> bucket(16);
+    for (uint32_t i = 0; i < fits; ++i) {
+        const bf_fit_desc &d = fit_table[i];
+        if (d.model > 3 || d.grid >= grids || d.config >= configs ||
+            d.points != grid_table[d.grid].points)
+            return "fit " + std::to_string(i) +
+                   ": model, grid or config out of range, or M differs from its grid";
+        if (d.p0_row == 0xffffffffu && d.points > 13000)
+            return "the device initial guess supports M up to 13,000 points; pass p0";
+        out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;
+        out.any_p0 = out.any_p0 || d.p0_row != 0xffffffffu;
+        out.out_rows = std::max(out.out_rows, d.out_row + 1);
+        bucket[d.model * 4 + ragged_class_of(d.points)].push_back(i);
+    }
+    out.order.reserve(fits);
+    for (int b = 0; b < 16; ++b) {
+        auto &v = bucket[b];
+        if (v.empty())
+            continue;
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t c) {
+            return fit_table[a].points > fit_table[c].points;
+        });
+        RaggedRun r{uint32_t(b / 4), b % 4, uint32_t(out.order.size()), uint32_t(v.size()), 0, 0};
+        for (uint32_t i : v) {
+            r.guesses += fit_table[i].p0_row == 0xffffffffu;
+            r.max_points = std::max(r.max_points, fit_table[i].points);
+        }
+        out.order.insert(out.order.end(), v.begin(), v.end());
+        out.runs.push_back(r);
+    }
+    return {};
+}
+
+hipError_t ragged_prepare(const RaggedLayout &layout, bool mixed) {
+    hipFuncAttributes attr;
+    hipError_t e;
+    if ((e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&reset_kernel))) != hipSuccess ||
+        (e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&grid_table_kernel))) !=
+            hipSuccess)
+        return e;
+    for (const RaggedRun &r : layout.runs) {
+        const bool guess = r.guesses != 0;
+        switch (r.model) {
+        case 0: e = prepare_ragged<0>(r.cls, mixed, guess); break;
+        case 1: e = prepare_ragged<1>(r.cls, mixed, guess); break;
+        case 2: e = prepare_ragged<2>(r.cls, mixed, guess); break;
+        case 3: e = prepare_ragged<3>(r.cls, mixed, guess); break;
+        default: e = hipErrorInvalidValue;
+        }
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, bool mixed,
+                          const float *x, const float *y, const float *p0, float *params,
+                          int32_t *status, uint32_t *iterations, float *chi2, float *cov,
+                          float *stderr_, int32_t *ustatus, hipStream_t s,
+                          const hipEvent_t *events) {
+    hipError_t e;
+    if (events && (e = hipEventRecord(events[0], s)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, dev.cursors,
+                       uint32_t(layout.runs.size()));
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(grid_table_kernel, dim3(dev.grid_count), dim3(256), 0, s, dev.grids, x,
+                       dev.info);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    for (const RaggedRun &r : layout.runs) {
+        if (!r.guesses)
+            continue;
+        const uint32_t count = r.max_points / 10 > 1 ? r.max_points / 10 : 1;
+        const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
+        if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset, r.count, dev.grids,
+                              dev.info, x, y, dev.guess, r.max_points, sort_n, s)) != hipSuccess)
+            return e;
+    }
+    if (events && (e = hipEventRecord(events[1], s)) != hipSuccess)
+        return e;
+    for (size_t i = 0; i < layout.runs.size(); ++i) {
+        const RaggedRun &r = layout.runs[i];
+        RaggedArgs a{};
+        a.fits = dev.fits;
+        a.grids = dev.grids;
+        a.grid_info = dev.info;
+        a.configs = dev.configs;
+        a.order = dev.order + r.offset;
+        a.count = r.count;
+        a.cursor = dev.cursors + i;
+        a.x = x;
+        a.y = y;
+        a.p0 = p0;
+        a.guess = dev.guess;
+        a.params = params;
+        a.status = status;
+        a.iterations = iterations;
+        a.chi2 = chi2;
+        a.cov = cov;
+        a.stderr_ = stderr_;
+        a.ustatus = ustatus;
+        const int G = kRaggedClasses[r.cls].G;
+        const uint32_t per_wave = 64 / uint32_t(G);
+        const uint32_t waves = (r.count + per_wave - 1) / per_wave;
+        const uint32_t needed = (waves + 3) / 4;
+        const uint32_t blocks = std::min<uint32_t>(needed, 256u * 8u);
+        if ((e = ragged_lm(r.model, r.cls, mixed, a, blocks, s)) != hipSuccess)
+            return e;
+    }
+    if (events && (e = hipEventRecord(events[2], s)) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
 
 }  // namespace bfhip
 
@@ -73,7 +182,7 @@ struct bf_ragged_plan_s {
     hipStream_t stream = nullptr;
     uint32_t fits = 0, grids = 0, configs = 0;
     bool mixed = false, uncertainties = false, any_guess = false, any_p0 = false;
-    uint32_t out_rows = 0;
+    RaggedLayout layout;
     bf_fit_desc *d_fits = nullptr;
     bf_grid_desc *d_grids = nullptr;
     bf_config *d_configs = nullptr;
@@ -81,11 +190,10 @@ struct bf_ragged_plan_s {
     uint32_t *d_order = nullptr;
     uint32_t *d_cursors = nullptr;
     float *d_guess = nullptr;
-    std::vector<RaggedClassRun> runs;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void *key[10] = {};
-    hipEvent_t ev[3] = {};
+    hipEvent_t ev[2] = {};
     bool timed = false;
 };
 
@@ -120,52 +228,13 @@ template <typename T> hipError_t upload(T **dst, const T *src, size_t n, hipStre
 int enqueue(bf_ragged_plan p, const float *x, const float *y, const float *p0, float *params,
             int32_t *status, uint32_t *iterations, float *chi2, float *cov, float *se,
             int32_t *us, char *err, size_t errlen) {
-    hipStream_t s = p->stream;
-    BF_TRY(hipEventRecord(p->ev[0], s), "hipEventRecord");
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, p->d_cursors,
-                       uint32_t(p->runs.size()));
-    BF_TRY(hipGetLastError(), "queue reset launch");
-    hipLaunchKernelGGL(grid_table_kernel, dim3(p->grids), dim3(256), 0, s, p->d_grids, x,
-                       p->d_info);
-    BF_TRY(hipGetLastError(), "grid table launch");
-    for (const RaggedClassRun &r : p->runs) {
-        if (!r.guesses)
-            continue;
-        const uint32_t count = r.max_points / 10 > 1 ? r.max_points / 10 : 1;
-        const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
-        BF_TRY(ragged_guess(r.model, p->d_fits, p->d_order + r.offset, r.count, p->d_grids,
-                            p->d_info, x, y, p->d_guess, r.max_points, sort_n, s),
-               "ragged guess launch");
-    }
-    BF_TRY(hipEventRecord(p->ev[1], s), "hipEventRecord");
-    for (size_t i = 0; i < p->runs.size(); ++i) {
-        const RaggedClassRun &r = p->runs[i];
-        RaggedArgs a{};
-        a.fits = p->d_fits;
-        a.grids = p->d_grids;
-        a.grid_info = p->d_info;
-        a.configs = p->d_configs;
-        a.order = p->d_order + r.offset;
-        a.count = r.count;
-        a.cursor = p->d_cursors + i;
-        a.x = x;
-        a.y = y;
-        a.p0 = p0;
-        a.guess = p->d_guess;
-        a.params = params;
-        a.status = status;
-        a.iterations = iterations;
-        a.chi2 = chi2;
-        a.cov = p->uncertainties ? cov : nullptr;
-        a.stderr_ = se;
-        a.ustatus = us;
-        const int G = kRaggedClasses[r.cls].G;
-        const uint32_t groups_per_block = 256 / uint32_t(G);
-        const uint32_t needed = (r.count + groups_per_block - 1) / groups_per_block;
-        const uint32_t blocks = std::min<uint32_t>(needed, 256u * 8u);
-        BF_TRY(ragged_lm(r.model, r.cls, p->mixed, a, blocks, s), "ragged LM launch");
-    }
-    BF_TRY(hipEventRecord(p->ev[2], s), "hipEventRecord");
+    const RaggedDevice dev{p->d_fits, p->d_grids, p->d_configs, p->d_info,
+                           p->d_order, p->d_cursors, p->d_guess, p->grids};
+    // No events inside the capture: an event recorded on a capturing stream
+    // only marks a dependency point and is not replayed by the graph.
+    BF_TRY(ragged_enqueue(p->layout, dev, p->mixed, x, y, p0, params, status, iterations, chi2,
+                          p->uncertainties ? cov : nullptr, se, us, p->stream, nullptr),
+           "ragged batch enqueue");
     return BF_OK;
 }
 
@@ -186,56 +255,23 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if (numerics != 0 && numerics != (BF_COMPENSATED | BF_MIXED_FP64))
         return fail(err, errlen, BF_INVALID_ARGUMENT,
                     "ragged batches support numerics 0 (FP32) or 5 (compensated mixed)");
-    if (fits && (!fit_table || !grid_table || !config_table || !grids || !configs))
+    if (fits && !config_table)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "descriptor tables are required");
-    for (uint32_t g = 0; g < grids; ++g)
-        if (grid_table[g].points == 0)
-            return fail(err, errlen, BF_INVALID_ARGUMENT, "every grid needs M>0");
     auto p = new bf_ragged_plan_s();
+    const std::string why = ragged_layout(fits, fit_table, grids, grid_table, configs, p->layout);
+    if (!why.empty()) {
+        delete p;
+        return fail(err, errlen, BF_INVALID_ARGUMENT, why);
+    }
     p->stream = stream ? static_cast<hipStream_t>(stream) : rt.stream;
     p->fits = fits;
     p->grids = grids;
     p->configs = configs;
     p->mixed = numerics != 0;
     p->uncertainties = uncertainties != 0;
-    // Bucket by (model, class); longest first inside each bucket.
-    std::vector<std::vector<uint32_t>> bucket(16);
-    for (uint32_t i = 0; i < fits; ++i) {
-        const bf_fit_desc &d = fit_table[i];
-        if (d.model > 3 || d.grid >= grids || d.config >= configs ||
-            d.points != grid_table[d.grid].points) {
-            release(p);
-            return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "fit " + std::to_string(i) +
-                            ": model, grid or config out of range, or M differs from its grid");
-        }
-        if (d.p0_row == 0xffffffffu && d.points > 13000) {
-            release(p);
-            return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "the device initial guess supports M up to 13,000 points; pass p0");
-        }
-        p->any_guess = p->any_guess || d.p0_row == 0xffffffffu;
-        p->any_p0 = p->any_p0 || d.p0_row != 0xffffffffu;
-        p->out_rows = std::max(p->out_rows, d.out_row + 1);
-        bucket[d.model * 4 + ragged_class_of(d.points)].push_back(i);
-    }
-    std::vector<uint32_t> order;
-    order.reserve(fits);
-    for (int b = 0; b < 16; ++b) {
-        auto &v = bucket[b];
-        if (v.empty())
-            continue;
-        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t c) {
-            return fit_table[a].points > fit_table[c].points;
-        });
-        RaggedClassRun r{uint32_t(b / 4), b % 4, uint32_t(order.size()), uint32_t(v.size()), 0, 0};
-        for (uint32_t i : v) {
-            r.guesses += fit_table[i].p0_row == 0xffffffffu;
-            r.max_points = std::max(r.max_points, fit_table[i].points);
-        }
-        order.insert(order.end(), v.begin(), v.end());
-        p->runs.push_back(r);
-    }
+    p->any_guess = p->layout.any_guess;
+    p->any_p0 = p->layout.any_p0;
+    const std::vector<uint32_t> &order = p->layout.order;
     hipStream_t s = p->stream;
     auto bad = [&](hipError_t e, const char *what) {
         release(p);
@@ -257,12 +293,14 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if ((e = hipMalloc(reinterpret_cast<void **>(&p->d_cursors), sizeof(uint32_t) * 16)) != hipSuccess)
         return bad(e, "hipMalloc(queue counters)");
     if (p->any_guess &&
-        (e = hipMalloc(reinterpret_cast<void **>(&p->d_guess), size_t(p->out_rows) * 8 * sizeof(float))) !=
-            hipSuccess)
+        (e = hipMalloc(reinterpret_cast<void **>(&p->d_guess),
+                       size_t(p->layout.out_rows) * 8 * sizeof(float))) != hipSuccess)
         return bad(e, "hipMalloc(guess rows)");
     for (hipEvent_t &ev : p->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess)
             return bad(e, "hipEventCreate");
+    if ((e = ragged_prepare(p->layout, p->mixed)) != hipSuccess)
+        return bad(e, "loading the ragged kernels");
     *plan = p;
     return BF_OK;
 }
@@ -307,7 +345,9 @@ int bf_hip_ragged_run(bf_ragged_plan p, const float *d_x, const float *d_y, cons
         BF_TRY(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         std::copy(key, key + 10, p->key);
     }
+    BF_TRY(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
     BF_TRY(hipGraphLaunch(p->exec, p->stream), "hipGraphLaunch");
+    BF_TRY(hipEventRecord(p->ev[1], p->stream), "hipEventRecord");
     p->timed = true;
     return BF_OK;
 }
@@ -317,14 +357,12 @@ int bf_hip_ragged_timing(bf_ragged_plan p, double *times, int count) {
         times[i] = 0.0;
     if (!p || !p->timed)
         return BF_INVALID_ARGUMENT;
-    if (hipEventSynchronize(p->ev[2]) != hipSuccess)
+    if (hipEventSynchronize(p->ev[1]) != hipSuccess)
         return BF_RUNTIME_ERROR;
-    for (int i = 0; i < count && i < 2; ++i) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]) != hipSuccess)
-            return BF_RUNTIME_ERROR;
-        times[i] = ms;
-    }
+    float ms = 0.0f;
+    if (count < 1 || hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) != hipSuccess)
+        return BF_RUNTIME_ERROR;
+    times[0] = ms;
     return BF_OK;
 }
 

@@ -507,10 +507,16 @@ int bf_hip_malloc(void **ptr, size_t bytes, char *err, size_t errlen) {
 int bf_hip_free(void *ptr) { return hipFree(ptr) == hipSuccess ? BF_OK : BF_RUNTIME_ERROR; }
 
 int bf_hip_memcpy(void *dst, const void *src, size_t bytes, int kind, char *err, size_t errlen) {
+    Runtime &rt = Runtime::get();
+    if (rt.status != BF_OK)
+        return fail(err, errlen, BF_UNAVAILABLE, "HIP is unavailable: " + rt.why);
     const hipMemcpyKind k = kind == 1   ? hipMemcpyHostToDevice
                             : kind == 2 ? hipMemcpyDeviceToHost
                                         : hipMemcpyDeviceToDevice;
-    BF_TRY(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+    // Ordered after the work already on the library stream (which is
+    // non-blocking, so a null-stream copy would not wait for it).
+    BF_TRY(hipMemcpyAsync(dst, src, bytes, k, rt.stream), "hipMemcpyAsync");
+    BF_TRY(hipStreamSynchronize(rt.stream), "hipStreamSynchronize");
     return BF_OK;
 }
 

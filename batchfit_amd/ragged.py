@@ -164,32 +164,17 @@ def fit_ragged(batches: Sequence, *, uncertainties: bool = False, numerics: Opti
                max_iterations: int = 100) -> List:
     """Fit several batches ``(x[M], y[N,M] or y[M], model, p0, bounds)`` in one
     ragged launch; returns one FitResult (or FitResultWithUncertainties) per batch."""
-    from . import FitResult, FitResultWithUncertainties, _PARAMETERS
+    from . import (FitResult, FitResultWithUncertainties, NUMERICS_DEFAULT, _make_config,
+                   _validate_batch)
     if not batches:
         return []
+    numerics = NUMERICS_DEFAULT if numerics is None else int(numerics)
     xs, ys, p0s, descs, grids, configs, shapes = [], [], [], [], [], [], []
     x_off = y_off = row = p0_rows = 0
     for b, (x, y, model, p0, bounds) in enumerate(batches):
-        if model not in _PARAMETERS:
-            raise ValueError(f"unknown model {model!r}")
-        x = np.ascontiguousarray(x, np.float32)
-        y = np.asarray(y, np.float32)
-        if y.ndim == 1:
-            y = y.reshape(1, -1)
-        if x.ndim != 1 or not x.size or y.ndim != 2 or y.shape[1] != x.size:
-            raise ValueError("each batch needs x[M] and y[N,M] with M>0")
-        n, m, p = y.shape[0], x.size, _PARAMETERS[model]
-        cfg = _native.default_config()
-        cfg.max_iterations = int(max_iterations)
-        cfg.numerics = 5 if numerics is None else int(numerics)
-        if bounds is not None:
-            lo = np.broadcast_to(np.asarray(bounds[0], np.float32), (p,))
-            hi = np.broadcast_to(np.asarray(bounds[1], np.float32), (p,))
-            if (np.any(np.isnan(lo)) or np.any(np.isnan(hi)) or np.any(lo > hi)
-                    or np.any(lo == np.inf) or np.any(hi == -np.inf)):
-                raise ValueError("invalid parameter bounds")
-            for k in range(p):
-                cfg.lower[k], cfg.upper[k] = float(lo[k]), float(hi[k])
+        x, y, p0, lower, upper = _validate_batch(x, y, model, p0, bounds)
+        n, m, p = y.shape[0], x.size, len(lower)
+        cfg = _make_config(n, m, lower, upper, max_iterations, numerics)
         configs.append(cfg)
         grids.append((x_off, m, 0))
         xs.append(x)

@@ -37,7 +37,8 @@ enum bf_error {
     BF_OK = 0,
     BF_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
     BF_RUNTIME_ERROR = 2,    /* a HIP call failed; message names it */
-    BF_UNAVAILABLE = 3       /* no gfx950 device visible */
+    BF_UNAVAILABLE = 3,      /* no gfx950 device visible */
+    BF_TIMEOUT = 4           /* bf_serve_wait: not finished within the timeout */
 };
 
 enum bf_model { BF_GAUSSIAN = 0, BF_LORENTZIAN = 1, BF_PSEUDO_VOIGT = 2, BF_TWO_LORENTZIAN = 3 };
@@ -132,8 +133,8 @@ int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *lau
  * descriptor names a p0 row) and outputs with rows of 8 (params, stderr), 64
  * (cov) or 1 (status, iterations, chi2, ustatus); cov/stderr/ustatus only when
  * the plan was created with uncertainties. The captured graph is reused while
- * the pointers stay the same. No counterpart in the reference (north-star
- * batching); each fit is solved as a uniform batch of one would be. */
+ * the pointers stay the same. No counterpart in the reference (it fits one
+ * uniform batch per call); each fit is solved as a uniform batch of one would be. */
 typedef struct bf_ragged_plan_s *bf_ragged_plan;
 int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc *fit_table,
                          uint32_t grids, const bf_grid_desc *grid_table, uint32_t configs,
@@ -143,9 +144,72 @@ int bf_hip_ragged_run(bf_ragged_plan plan, const float *d_x, const float *d_y, c
                       float *d_params, int32_t *d_status, uint32_t *d_iterations, float *d_chi2,
                       float *d_cov, float *d_stderr, int32_t *d_ustatus, char *err,
                       size_t errlen);
-/* Device timing of the last run of the plan (ms): [0] prologue + guess, [1] LM. */
+/* Device time of the last run of the plan (ms), measured by events recorded
+ * on the plan's stream around the graph launch: [0] whole graph; [1..] 0. */
 int bf_hip_ragged_timing(bf_ragged_plan plan, double *times, int count);
 int bf_hip_ragged_destroy(bf_ragged_plan plan);
+
+/* Continuous-batching server. Any host thread submits requests (one model,
+ * one grid x[M], N spectra y[N*M], optional p0[N*P], its own bounds and
+ * options in cfg with cfg->fits = N, cfg->points = M); a scheduler thread
+ * admits pending requests in arrival order into free device slots while the
+ * bytes held by admitted batches stay within hbm_budget, and runs each
+ * admitted batch as one ragged launch on that slot's stream, so up to
+ * `slots` batches overlap on the device. Host-array requests are staged
+ * through pinned memory (inputs are borrowed until bf_serve_wait returns);
+ * on_device requests name device arrays and receive device outputs (no
+ * uncertainties). A request's result equals the same request alone. No
+ * counterpart in the reference (it serialises fitting behind one mutex,
+ * python/bindings.cpp:13,66, and streams NPY through a 3-slot ring,
+ * src/metal/pipeline.cpp:99-169). */
+typedef struct bf_server_s *bf_server;
+typedef struct bf_serve_options {
+    uint64_t hbm_budget;     /* bytes admitted batches may hold; 0 = 64 GiB */
+    uint32_t slots;          /* batches on the device at once; 0 = 3 (max 16) */
+    uint32_t max_batch_fits; /* fits per admitted batch; 0 = 4,194,304 */
+    uint32_t numerics;       /* 0 (FP32) or 5 (compensated mixed, default) */
+    uint32_t reserved;
+} bf_serve_options;
+typedef struct bf_request_info {
+    int32_t rc;        /* BF_OK or the batch's error */
+    uint32_t batch;    /* admitted batch number */
+    double queued_s;   /* submit → admission */
+    double service_s;  /* admission → completion */
+    double latency_s;  /* submit → completion */
+    char message[256];
+} bf_request_info;
+typedef struct bf_serve_counters {
+    uint64_t requests_done, fits_done, batches, pending;
+    uint64_t bytes_in_flight, max_bytes_in_flight;
+    uint32_t max_batches_in_flight, reserved;
+} bf_serve_counters;
+int bf_serve_create(bf_server *server, const bf_serve_options *options, char *err, size_t errlen);
+int bf_serve_submit(bf_server server, uint32_t model, const bf_config *cfg, int on_device,
+                    const float *x, const float *y, const float *p0, float *params,
+                    int32_t *status, uint32_t *iterations, float *chi2, float *cov,
+                    float *stderr_, int32_t *ustatus, uint64_t *ticket, char *err, size_t errlen);
+/* Wait for a request (timeout_s < 0 waits forever); returns the request's rc
+ * or BF_TIMEOUT. The ticket is released once it returns anything but BF_TIMEOUT. */
+int bf_serve_wait(bf_server server, uint64_t ticket, double timeout_s, bf_request_info *info);
+int bf_serve_stats(bf_server server, bf_serve_counters *stats);
+/* Drain every admitted and pending request, then stop the threads. */
+int bf_serve_destroy(bf_server server);
+
+/* Seeded synthetic spectra written straight into HBM (benchmarks, serving
+ * load): y[n*m] (fit-major) of `model` on the device grid d_x[m], parameters
+ * per fit drawn uniformly from [lower[k], upper[k]) (8 entries), noise
+ * N(0, σ²) with σ = noise·amplitude (noise_relative) or σ = noise. layout 1
+ * (two_lorentzian) draws one HWHM for both peaks and sets c2 = c1 +
+ * U[sep_lo, sep_hi)·HWHM. Philox4x32-10 counters (seed, fit_offset + fit,
+ * point) make every fit independent of how the batch is split. d_truth[n*8]
+ * receives the drawn parameters when not NULL. Enqueued on `stream`. No
+ * counterpart in the reference's hot path (its generator is
+ * src/cpu/synthetic.cpp on the host). */
+int bf_hip_synthesize(uint32_t model, uint32_t n, uint32_t m, const float *d_x,
+                      const float *lower, const float *upper, uint32_t layout, float sep_lo,
+                      float sep_hi, uint32_t noise_relative, float noise, uint64_t seed,
+                      uint64_t fit_offset, float *d_y, float *d_truth, void *stream, char *err,
+                      size_t errlen);
 
 /* Data-only initial guess alone, host arrays (cpu/initial_guess.hpp:120-249):
  * params[N*P], baseline[N], gstatus[N] (GuessStatus codes 0..3). */
@@ -166,7 +230,8 @@ int bf_hip_set_timing(int enable);
 int bf_hip_last_timing(double *times, int count);
 
 /* Device memory helpers so hosts without another HIP binding can stage
- * inputs in HBM: plain hipMalloc/hipFree/hipMemcpy (kind 1 H2D, 2 D2H, 3 D2D). */
+ * inputs in HBM: hipMalloc, hipFree and a synchronous copy (kind 1 H2D, 2 D2H,
+ * 3 D2D) ordered after the work already enqueued on the library's stream. */
 int bf_hip_malloc(void **ptr, size_t bytes, char *err, size_t errlen);
 int bf_hip_free(void *ptr);
 int bf_hip_memcpy(void *dst, const void *src, size_t bytes, int kind, char *err, size_t errlen);

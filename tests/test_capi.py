@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "batchfit_hip.h")).read()
-    return sorted(set(re.findall(r"\b(bf_hip_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(bf_(?:hip|serve)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -96,3 +96,27 @@ def test_no_oracle_in_product_path():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in text.lower(), f
+
+
+def test_server_and_plans_need_a_device():
+    """Without a usable gfx950 device the server and ragged plans refuse to start
+    (RuntimeError), and the serve entries reject null handles."""
+    if batchfit_amd.hip_available():
+        pytest.skip("a device is present")
+    from batchfit_amd.ragged import fit_ragged
+    from batchfit_amd.serve import Server
+    with pytest.raises(RuntimeError):
+        Server()
+    x = np.linspace(-1, 1, 32, dtype=np.float32)
+    with pytest.raises(RuntimeError):
+        fit_ragged([(x, np.ones((2, 32), np.float32), "gaussian", None, None)])
+    lib = _native.library()
+    import batchfit_amd.serve as serve
+    serve._setup(lib)
+    ticket = ctypes.c_uint64()
+    err = _native.error_buffer()
+    cfg = _native.default_config()
+    assert lib.bf_serve_submit(None, 0, ctypes.byref(cfg), 0, *([None] * 10), ctypes.byref(ticket),
+                               err, len(err)) == _native.BF_INVALID_ARGUMENT
+    assert lib.bf_serve_wait(None, 1, 0.0, None) == _native.BF_INVALID_ARGUMENT
+    assert lib.bf_serve_destroy(None) == _native.BF_OK
