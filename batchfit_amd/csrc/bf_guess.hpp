@@ -384,31 +384,39 @@ __device__ __forceinline__ int raw_peak_dev(const float *s, int i) {
     return best;
 }
 
-// Prominence of the smoothed maximum at j (initial_guess.hpp:38-54).
-__device__ float prominence_dev(const float *s, int m, int j, int lane) {
-    const float height = smooth_at(s, m, j);
-    int kl = -1, kr = 0x7fffffff;
-    for (int k = lane; k < j; k += 64)
-        if (smooth_at(s, m, k) > height)
-            kl = k;
-    for (int k = j + 1 + lane; k < m; k += 64)
-        if (smooth_at(s, m, k) > height && k < kr)
-            kr = k;
-    kl = wave_max_i(kl);
-    kr = wave_min_i(kr);
-    float left = height, right = height;
-    for (int k = kl + 1 + lane; k < j; k += 64) {
+// Whether the smoothed maximum at j of height h has prominence > t
+// (initial_guess.hpp:38-54 and the `<= 3·noise` test of :212), decided by
+// one lane. The reference walks each side to the first higher point, takes
+// the side minima L and R and compares h − max(L, R) with t. Rounding of
+// h − v is monotone in v, so the test holds iff h − L > t and h − R > t, and
+// each side can stop as soon as its running minimum passes: a walk ends at
+// the first higher point, at the first point deep enough, or at the edge,
+// instead of always reaching the first higher point.
+__device__ bool prominent_dev(const float *s, int m, int j, float h, float t) {
+    float lo = h;
+    bool left = false;
+    for (int k = j - 1; k >= 0; --k) {
         const float v = smooth_at(s, m, k);
-        left = (v < left) ? v : left;
+        if (v > h)
+            break;
+        lo = (v < lo) ? v : lo;
+        if (h - lo > t) {
+            left = true;
+            break;
+        }
     }
-    const int stop = kr == 0x7fffffff ? m : kr;
-    for (int k = j + 1 + lane; k < stop; k += 64) {
+    if (!left)
+        return false;
+    lo = h;
+    for (int k = j + 1; k < m; ++k) {
         const float v = smooth_at(s, m, k);
-        right = (v < right) ? v : right;
+        if (v > h)
+            break;
+        lo = (v < lo) ? v : lo;
+        if (h - lo > t)
+            return true;
     }
-    left = wave_min_f(left);
-    right = wave_min_f(right);
-    return height - ((left < right) ? right : left);
+    return false;
 }
 
 // The guess for fit `fit` of `a`, run by one whole wave.
@@ -522,39 +530,28 @@ template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit)
         const float lr = primary.left_radius, rr = primary.right_radius;
         const float a3 = 3.0f * grid.guess_spacing, a2 = 2.0f * ((rr < lr) ? rr : lr);
         const float separation = (a3 < a2) ? a2 : a3;
-        // Candidates in decreasing (value, −index) order until one is
-        // prominent enough (initial_guess.hpp:203-217).
-        float prev_v = __uint_as_float(0x7f800000u);
-        int prev_j = -1;
-        int second = -1;
-        for (;;) {
-            float v_best = 0.0f;
-            int j_best = 0x7fffffff;
-            for (int j = 1 + lane; j + 1 < m; j += 64) {
-                const float v = smooth_at(s, m, j);
-                if (!(v > 0.0f) || v <= smooth_at(s, m, j - 1) || v < smooth_at(s, m, j + 1))
-                    continue;
-                if (!(v < prev_v || (v == prev_v && j > prev_j)))
-                    continue;
-                const int idx = raw_peak_dev(s, j);
-                const int bins = idx > first ? idx - first : first - idx;
-                if (bins < 3 || fabsf(x[idx] - primary.center) < separation)
-                    continue;
-                if (v > v_best || (v == v_best && j < j_best)) {
-                    v_best = v;
-                    j_best = j;
-                }
-            }
-            wave_argmax(v_best, j_best);
-            if (j_best == 0x7fffffff)
-                break;
-            if (prominence_dev(s, m, j_best, lane) > 3.0f * noise) {
-                second = raw_peak_dev(s, j_best);
-                break;
-            }
-            prev_v = v_best;
-            prev_j = j_best;
+        // The prominent local maximum of largest smoothed value, earliest on
+        // ties (initial_guess.hpp:203-217): every lane tests its own
+        // candidates (only those that would beat its best so far), then one
+        // wave reduction picks (largest value, smallest index).
+        const float t = 3.0f * noise;
+        float v_best = 0.0f;
+        int j_best = 0x7fffffff;
+        for (int j = 1 + lane; j + 1 < m; j += 64) {
+            const float v = smooth_at(s, m, j);
+            if (!(v > v_best) || v <= smooth_at(s, m, j - 1) || v < smooth_at(s, m, j + 1))
+                continue;
+            const int idx = raw_peak_dev(s, j);
+            const int bins = idx > first ? idx - first : first - idx;
+            if (bins < 3 || fabsf(x[idx] - primary.center) < separation)
+                continue;
+            if (!prominent_dev(s, m, j, v, t))
+                continue;
+            v_best = v;
+            j_best = j;
         }
+        wave_argmax(v_best, j_best);
+        int second = j_best == 0x7fffffff ? -1 : raw_peak_dev(s, j_best);
         if (second < 0)
             return finish();
         if (first > second) {

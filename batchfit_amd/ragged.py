@@ -32,11 +32,22 @@ class DeviceArray:
         self.lib = _native.library()
         self.nbytes = int(nbytes)
         self.ptr = ctypes.c_void_p()
+        self._owner = True
+        self._base = None
         err = _native.error_buffer()
         _native.check(self.lib.bf_hip_malloc(ctypes.byref(self.ptr), max(1, self.nbytes), err,
                                              len(err)), err)
         if host is not None:
             self.upload(host)
+
+    def view(self, offset: int, nbytes: int) -> "DeviceArray":
+        """A non-owning window [offset, offset+nbytes) of this buffer."""
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view exceeds the device buffer")
+        v = object.__new__(DeviceArray)
+        v.lib, v.nbytes, v._owner, v._base = self.lib, int(nbytes), False, self
+        v.ptr = ctypes.c_void_p(self.ptr.value + int(offset))
+        return v
 
     @classmethod
     def of(cls, host: np.ndarray) -> "DeviceArray":
@@ -62,9 +73,9 @@ class DeviceArray:
         return host
 
     def free(self) -> None:
-        if self.ptr.value:
+        if self._owner and self.ptr.value:
             self.lib.bf_hip_free(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        self.ptr = ctypes.c_void_p()
 
     def __del__(self):
         try:

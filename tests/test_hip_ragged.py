@@ -88,7 +88,8 @@ def test_ragged_uncertainties_and_failures(hip, oracle_port):
 
 def test_graph_replays_match_eager(hip):
     from batchfit_amd import ragged
-    batches = mixed_batches()[:8]
+    # Every fit starts from the device guess, as in the plan built below.
+    batches = [(x, y, model, None, bounds) for x, y, model, _, bounds in mixed_batches()[:8]]
     first = ragged.fit_ragged(batches)
     # Build one plan and replay its graph several times with eager work between.
     import batchfit_amd._native as native
