@@ -20,11 +20,23 @@ struct GuessArgs {
     const GridInfo *grid;
     uint32_t n, m;
     uint32_t m_pad;   // LDS floats reserved for the signal
-    uint32_t sort_n;  // LDS floats reserved for sorting windows longer than 128
+    uint32_t sort_n;  // LDS floats reserved for sorting (guess_sort_floats)
     float *params;    // [N,P]
     float *baseline;  // [N] or null
     int32_t *gstatus; // [N] or null
 };
+
+// LDS floats the guess needs beyond the signal for an edge window of `count`
+// points: medians of up to 1024 values are selected in registers, so only
+// windows whose union exceeds that (M > 5129) sort in LDS.
+__host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
+    if (2 * count <= 1024)
+        return 0;
+    uint32_t n = 1;
+    while (n < 2 * count)
+        n <<= 1;
+    return n;
+}
 
 enum : int32_t { GS_SUCCESS = 0, GS_INVALID = 1, GS_NO_PEAK = 2, GS_INSUFFICIENT = 3 };
 
@@ -222,13 +234,19 @@ template <int SLOTS, int K> __device__ __forceinline__ void bitonic_sort_to(floa
     bitonic_merge<SLOTS, K, K / 2>(v, lane);
 }
 
+// Element k (wave-uniform) of the register-held sequence e = q·64 + lane.
+// Every slot is read with readlane first and the slot is chosen among the
+// scalar results: a select among the slots themselves is turned into an
+// indexed load by the optimiser, which moves the whole array to scratch.
 template <int SLOTS> __device__ __forceinline__ float reg_pick(const float (&v)[SLOTS], int k) {
     const int q = k >> 6, lane = k & 63;
-    float w = v[0];
+    int w = __builtin_amdgcn_readlane(__float_as_int(v[0]), lane);
 #pragma unroll
-    for (int i = 1; i < SLOTS; ++i)
-        w = q == i ? v[i] : w;
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), lane));
+    for (int i = 1; i < SLOTS; ++i) {
+        const int r = __builtin_amdgcn_readlane(__float_as_int(v[i]), lane);
+        w = q == i ? r : w;
+    }
+    return __int_as_float(w);
 }
 
 template <int SLOTS, typename F>
@@ -244,8 +262,64 @@ __device__ __forceinline__ float reg_median(int count, int lane, F &&get) {
                      : std_midpoint(reg_pick<SLOTS>(v, count / 2 - 1), reg_pick<SLOTS>(v, count / 2));
 }
 
+// Order-preserving 32-bit key of a finite float (-0 sorts just below +0) and
+// its inverse.
+__device__ __forceinline__ uint32_t float_key(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+// Number of lanes of the wave for which `p` holds (a scalar).
+__device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p)); }
+
+// Median of count ≤ 64·SLOTS finite values get(0..count) by exact selection
+// (initial_guess.hpp:25-30: the middle order statistic, and for an even count
+// the midpoint of it and the largest value below it). The keys stay in
+// registers; the k-th smallest key is found bit by bit from the top, each
+// step counting the keys at or below a bound with one ballot per slot.
+template <int SLOTS, typename F>
+__device__ float select_median(int count, int lane, F &&get) {
+    uint32_t key[SLOTS];
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        const int e = q * 64 + lane;
+        key[q] = e < count ? float_key(get(e)) : 0xffffffffu;  // above every finite key
+    }
+    const int k = count / 2;
+    uint32_t t = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t low = t | ((1u << b) - 1u);
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < SLOTS; ++q)
+            c += wave_count(key[q] <= low);
+        if (c < k + 1)
+            t |= 1u << b;
+    }
+    if (count % 2)
+        return key_float(t);
+    int below = 0;
+    uint32_t under = 0;
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+        below += wave_count(key[q] < t);
+        under = (key[q] < t && key[q] > under) ? key[q] : under;
+    }
+    uint32_t lower = t;
+    if (below >= k) {  // the k smallest are all below t: the largest of them
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t w = uint32_t(__shfl_xor(int(under), o, 64));
+            under = w > under ? w : under;
+        }
+        lower = under;
+    }
+    return std_midpoint(key_float(lower), key_float(t));
+}
+
 // Median of get(0..count) (initial_guess.hpp:25-30): register sorts up to 256
-// values, an LDS sort in buf beyond.
+// values, exact selection up to 1024, an LDS sort in buf beyond.
 template <typename F>
 __device__ float wave_median(float *buf, int count, int lane, F &&get) {
     if (count <= 64)
@@ -254,6 +328,10 @@ __device__ float wave_median(float *buf, int count, int lane, F &&get) {
         return reg_median<2>(count, lane, get);
     if (count <= 256)
         return reg_median<4>(count, lane, get);
+    if (count <= 512)
+        return select_median<8>(count, lane, get);
+    if (count <= 1024)
+        return select_median<16>(count, lane, get);
     const int n = pow2_at_least(count);
     for (int i = lane; i < n; i += 64)
         buf[i] = i < count ? get(i) : __uint_as_float(0x7f800000u);

@@ -38,10 +38,28 @@ struct RaggedArgs {
     int32_t *ustatus;
 };
 
+// The value of lane 0 in every lane, held in scalar registers (for data a
+// whole wave shares: with 64-lane groups a descriptor and its config are
+// per-wave, and scalar copies keep them out of the vector register budget).
+template <typename T> __device__ __forceinline__ T wave_uniform(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i)
+        w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // Per-fit FitArgs over pre-offset pointers, so solve_fit works on fit 0.
+template <int G>
 __device__ __forceinline__ FitArgs ragged_fit_args(const RaggedArgs &r, const bf_fit_desc &d,
                                                    uint32_t model, int p) {
-    const bf_config &c = r.configs[d.config];
+    bf_config c = r.configs[d.config];
+    if constexpr (G == 64)
+        c = wave_uniform(c);
     FitArgs a;
     a.x = r.x + r.grids[d.grid].x_offset;
     a.y = r.y + d.y_offset;
@@ -94,14 +112,37 @@ __device__ __forceinline__ FitArgs ragged_fit_args(const RaggedArgs &r, const bf
 // reduction on a path that skipped the queue read: with a per-group exit, the
 // optimiser may thread the lanes with gl != 0 straight back to the group
 // broadcast, which then reads lane 0 while it is inactive.
+// Orders the LDS accesses of one wave around a hand-over between its lanes
+// (the LDS unit runs one wave's instructions in order; this keeps the
+// compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Occupancy floor of the persistent kernels: the work-queue loop and the
+// per-fit descriptor cost a few registers more than the uniform kernel, so
+// where that kernel asks for three waves per SIMD this one asks for two
+// (three would spill).
+constexpr int lm_ragged_min_waves(int model, int g, int r) {
+    return lm_min_waves(model, g, r) > 2 ? 2 : lm_min_waves(model, g, r);
+}
+
 template <int MODEL, int G, int R, bool COMP, bool MIXED>
-__global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, R)) lm_ragged_kernel(RaggedArgs r) {
+__global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, R))
+    lm_ragged_kernel(RaggedArgs r) {
     using S = typename std::conditional<MIXED, double, float>::type;
     constexpr int P = Params<MODEL>::P;
     constexpr uint32_t per_wave = 64 / G;
     const int lane = int(threadIdx.x % 64);
     const int gl = lane % G;
     const uint32_t g = uint32_t(lane / G);
+    // With a register tile, each group copies its fit's grid into its own LDS
+    // slice of G·R floats (zero beyond M), as the uniform kernel does for the
+    // shared grid: the points then read x from LDS, not from memory.
+    __shared__ float xs[R > 0 ? 256 * R : 1];
+    float *const my_x = xs + (R > 0 ? (threadIdx.x / G) * G * R : 0);
     for (;;) {
         const uint32_t ticket = atomicAdd(r.cursor, lane == 0 ? 1u : 0u);
         const uint32_t base = __builtin_amdgcn_readfirstlane(ticket) * per_wave;
@@ -109,9 +150,22 @@ __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, R)) lm_ragged_kern
             break;
         const uint32_t task = base + g;
         if (task < r.count) {
-            const bf_fit_desc d = r.fits[r.order[task]];
-            const FitArgs a = ragged_fit_args(r, d, MODEL, P);
-            solve_fit<MODEL, G, R, COMP, S>(a, 0, gl, a.x);
+            bf_fit_desc d = r.fits[r.order[task]];
+            if constexpr (G == 64)
+                d = wave_uniform(d);
+            const FitArgs a = ragged_fit_args<G>(r, d, MODEL, P);
+            if constexpr (R > 0) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int j = gl + G * k;
+                    my_x[j] = j < int(a.m) ? a.x[j] : 0.0f;
+                }
+                wave_lds_sync();
+                solve_fit<MODEL, G, R, COMP, S>(a, 0, gl, my_x);
+                wave_lds_sync();  // every lane has read the slice before it is refilled
+            } else {
+                solve_fit<MODEL, G, R, COMP, S>(a, 0, gl, a.x);
+            }
         }
     }
 }
@@ -167,6 +221,7 @@ struct RaggedRun {
     uint32_t offset, count;  // slice of the order array
     uint32_t guesses;        // fits in the slice that start from the device guess
     uint32_t max_points;
+    double cost;             // relative work, for ordering and branch balance
 };
 
 // Host view of a ragged batch: validated descriptors bucketed by (model,
@@ -174,6 +229,7 @@ struct RaggedRun {
 struct RaggedLayout {
     std::vector<uint32_t> order;
     std::vector<RaggedRun> runs;
+    std::vector<size_t> cost_order;  // runs, heaviest first
     bool any_guess = false, any_p0 = false;
     uint32_t out_rows = 0;
 };
@@ -194,14 +250,32 @@ struct RaggedDevice {
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
                           const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out);
 
+// Device inputs and outputs of one ragged batch (rows of 8 for params and
+// stderr, 64 for cov; cov/stderr/ustatus null without uncertainties).
+struct RaggedIO {
+    const float *x, *y, *p0;
+    float *params;
+    int32_t *status;
+    uint32_t *iterations;
+    float *chi2, *cov, *stderr_;
+    int32_t *ustatus;
+};
+
+// Side streams and events for running the runs of a batch as parallel
+// branches (stream capture or eager).
+struct RaggedFork {
+    int streams;
+    hipStream_t *side;
+    hipEvent_t fork;
+    hipEvent_t *join;
+};
+
 // Enqueue a whole ragged batch on `stream` (no synchronisation, no capture):
-// queue reset, grid table, guesses, persistent LM kernels. `events` (3 or
-// null) are recorded before the prologue, before the LM and at the end.
+// queue reset and grid table on `stream`, then each run's guess and LM —
+// on `stream` in order of decreasing cost, or spread over fork->side
+// branches that join back into `stream`.
 hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, bool mixed,
-                          const float *x, const float *y, const float *p0, float *params,
-                          int32_t *status, uint32_t *iterations, float *chi2, float *cov,
-                          float *stderr_, int32_t *ustatus, hipStream_t stream,
-                          const hipEvent_t *events);
+                          const RaggedIO &io, hipStream_t stream, const RaggedFork *fork);
 
 // Load the kernels a layout launches (outside any capture).
 hipError_t ragged_prepare(const RaggedLayout &layout, bool mixed);

@@ -50,6 +50,38 @@ hipError_t ragged_guess(uint32_t model, const bf_fit_desc *fits, const uint32_t 
 
 }  // namespace
 
+namespace {
+
+// Stable sort of fit indices by M, largest first: a 16-bit counting sort per
+// pass (one pass when every M < 65,536) for large classes, so a batch of
+// millions of fits is laid out in linear time.
+void sort_longest_first(std::vector<uint32_t> &v, const bf_fit_desc *t) {
+    if (v.size() < 4096) {
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t c) {
+            return t[a].points > t[c].points;
+        });
+        return;
+    }
+    uint32_t top = 0;
+    for (uint32_t i : v)
+        top = std::max(top, t[i].points);
+    std::vector<uint32_t> tmp(v.size()), count(65537);
+    for (int shift = 0; shift < 32; shift += 16) {
+        if (shift && (top >> 16) == 0)
+            break;
+        std::fill(count.begin(), count.end(), 0u);
+        for (uint32_t i : v)
+            ++count[(((top - t[i].points) >> shift) & 0xffffu) + 1];
+        for (size_t d = 1; d < count.size(); ++d)
+            count[d] += count[d - 1];
+        for (uint32_t i : v)
+            tmp[count[((top - t[i].points) >> shift) & 0xffffu]++] = i;
+        v.swap(tmp);
+    }
+}
+
+}  // namespace
+
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
                           const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out) {
     out = RaggedLayout{};
@@ -77,17 +109,26 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         auto &v = bucket[b];
         if (v.empty())
             continue;
-        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t c) {
-            return fit_table[a].points > fit_table[c].points;
-        });
-        RaggedRun r{uint32_t(b / 4), b % 4, uint32_t(out.order.size()), uint32_t(v.size()), 0, 0};
+        sort_longest_first(v, fit_table);
+        RaggedRun r{uint32_t(b / 4), b % 4, uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0};
         for (uint32_t i : v) {
             r.guesses += fit_table[i].p0_row == 0xffffffffu;
             r.max_points = std::max(r.max_points, fit_table[i].points);
         }
+        // Relative work: points × parameters-weighted passes (a P=8 fit costs
+        // about three times a P=3 one per point).
+        static constexpr double kWeight[4] = {1.0, 1.0, 1.5, 3.5};
+        double points = 0.0;
+        for (uint32_t i : v)
+            points += fit_table[i].points;
+        r.cost = points * kWeight[r.model];
         out.order.insert(out.order.end(), v.begin(), v.end());
         out.runs.push_back(r);
     }
+    out.cost_order.resize(out.runs.size());
+    std::iota(out.cost_order.begin(), out.cost_order.end(), size_t(0));
+    std::stable_sort(out.cost_order.begin(), out.cost_order.end(),
+                     [&](size_t a, size_t b) { return out.runs[a].cost > out.runs[b].cost; });
     return {};
 }
 
@@ -113,64 +154,85 @@ hipError_t ragged_prepare(const RaggedLayout &layout, bool mixed) {
     return hipSuccess;
 }
 
-hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, bool mixed,
-                          const float *x, const float *y, const float *p0, float *params,
-                          int32_t *status, uint32_t *iterations, float *chi2, float *cov,
-                          float *stderr_, int32_t *ustatus, hipStream_t s,
-                          const hipEvent_t *events) {
+namespace {
+
+// Guess (when any fit of the run starts from it) and LM of run i on `s`.
+hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice &dev, bool mixed,
+                       const RaggedIO &io, hipStream_t s) {
+    const RaggedRun &r = layout.runs[i];
     hipError_t e;
-    if (events && (e = hipEventRecord(events[0], s)) != hipSuccess)
-        return e;
+    if (r.guesses) {
+        const uint32_t count = r.max_points / 10 > 1 ? r.max_points / 10 : 1;
+        const uint32_t sort_n = guess_sort_floats(count);
+        if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset, r.count, dev.grids,
+                              dev.info, io.x, io.y, dev.guess, r.max_points, sort_n, s)) !=
+            hipSuccess)
+            return e;
+    }
+    RaggedArgs a{};
+    a.fits = dev.fits;
+    a.grids = dev.grids;
+    a.grid_info = dev.info;
+    a.configs = dev.configs;
+    a.order = dev.order + r.offset;
+    a.count = r.count;
+    a.cursor = dev.cursors + i;
+    a.x = io.x;
+    a.y = io.y;
+    a.p0 = io.p0;
+    a.guess = dev.guess;
+    a.params = io.params;
+    a.status = io.status;
+    a.iterations = io.iterations;
+    a.chi2 = io.chi2;
+    a.cov = io.cov;
+    a.stderr_ = io.stderr_;
+    a.ustatus = io.ustatus;
+    const uint32_t per_wave = 64 / uint32_t(kRaggedClasses[r.cls].G);
+    const uint32_t waves = (r.count + per_wave - 1) / per_wave;
+    const uint32_t blocks = std::min<uint32_t>((waves + 3) / 4, 256u * 8u);
+    return ragged_lm(r.model, r.cls, mixed, a, blocks, s);
+}
+
+}  // namespace
+
+hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, bool mixed,
+                          const RaggedIO &io, hipStream_t s, const RaggedFork *fork) {
+    hipError_t e;
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, dev.cursors,
                        uint32_t(layout.runs.size()));
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(grid_table_kernel, dim3(dev.grid_count), dim3(256), 0, s, dev.grids, x,
+    hipLaunchKernelGGL(grid_table_kernel, dim3(dev.grid_count), dim3(256), 0, s, dev.grids, io.x,
                        dev.info);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    for (const RaggedRun &r : layout.runs) {
-        if (!r.guesses)
-            continue;
-        const uint32_t count = r.max_points / 10 > 1 ? r.max_points / 10 : 1;
-        const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
-        if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset, r.count, dev.grids,
-                              dev.info, x, y, dev.guess, r.max_points, sort_n, s)) != hipSuccess)
+    const int branches = fork ? std::min<int>(fork->streams, int(layout.runs.size())) : 0;
+    if (branches < 2) {
+        for (size_t k = 0; k < layout.runs.size(); ++k)
+            if ((e = enqueue_run(layout, layout.cost_order[k], dev, mixed, io, s)) != hipSuccess)
+                return e;
+        return hipSuccess;
+    }
+    // Fork: every branch waits for the prologue, runs its share of the runs
+    // (heaviest first, each to the least loaded branch) and joins back.
+    if ((e = hipEventRecord(fork->fork, s)) != hipSuccess)
+        return e;
+    for (int b = 0; b < branches; ++b)
+        if ((e = hipStreamWaitEvent(fork->side[b], fork->fork, 0)) != hipSuccess)
+            return e;
+    std::vector<double> load(branches, 0.0);
+    for (size_t k = 0; k < layout.runs.size(); ++k) {
+        const size_t i = layout.cost_order[k];
+        const int b = int(std::min_element(load.begin(), load.end()) - load.begin());
+        load[b] += layout.runs[i].cost;
+        if ((e = enqueue_run(layout, i, dev, mixed, io, fork->side[b])) != hipSuccess)
             return e;
     }
-    if (events && (e = hipEventRecord(events[1], s)) != hipSuccess)
-        return e;
-    for (size_t i = 0; i < layout.runs.size(); ++i) {
-        const RaggedRun &r = layout.runs[i];
-        RaggedArgs a{};
-        a.fits = dev.fits;
-        a.grids = dev.grids;
-        a.grid_info = dev.info;
-        a.configs = dev.configs;
-        a.order = dev.order + r.offset;
-        a.count = r.count;
-        a.cursor = dev.cursors + i;
-        a.x = x;
-        a.y = y;
-        a.p0 = p0;
-        a.guess = dev.guess;
-        a.params = params;
-        a.status = status;
-        a.iterations = iterations;
-        a.chi2 = chi2;
-        a.cov = cov;
-        a.stderr_ = stderr_;
-        a.ustatus = ustatus;
-        const int G = kRaggedClasses[r.cls].G;
-        const uint32_t per_wave = 64 / uint32_t(G);
-        const uint32_t waves = (r.count + per_wave - 1) / per_wave;
-        const uint32_t needed = (waves + 3) / 4;
-        const uint32_t blocks = std::min<uint32_t>(needed, 256u * 8u);
-        if ((e = ragged_lm(r.model, r.cls, mixed, a, blocks, s)) != hipSuccess)
+    for (int b = 0; b < branches; ++b)
+        if ((e = hipEventRecord(fork->join[b], fork->side[b])) != hipSuccess ||
+            (e = hipStreamWaitEvent(s, fork->join[b], 0)) != hipSuccess)
             return e;
-    }
-    if (events && (e = hipEventRecord(events[2], s)) != hipSuccess)
-        return e;
     return hipSuccess;
 }
 
@@ -195,6 +257,11 @@ struct bf_ragged_plan_s {
     const void *key[10] = {};
     hipEvent_t ev[2] = {};
     bool timed = false;
+    // Branches of the captured graph: every (model, class) run is a chain
+    // guess → LM on one of these streams, so the runs' tails overlap.
+    static constexpr int kBranches = 4;
+    hipStream_t side[kBranches] = {};
+    hipEvent_t fork = nullptr, join[kBranches] = {};
 };
 
 namespace {
@@ -214,6 +281,14 @@ void release(bf_ragged_plan p) {
     for (hipEvent_t e : p->ev)
         if (e)
             (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->join)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (p->fork)
+        (void)hipEventDestroy(p->fork);
+    for (hipStream_t st : p->side)
+        if (st)
+            (void)hipStreamDestroy(st);
     delete p;
 }
 
@@ -230,10 +305,12 @@ int enqueue(bf_ragged_plan p, const float *x, const float *y, const float *p0, f
             int32_t *us, char *err, size_t errlen) {
     const RaggedDevice dev{p->d_fits, p->d_grids, p->d_configs, p->d_info,
                            p->d_order, p->d_cursors, p->d_guess, p->grids};
-    // No events inside the capture: an event recorded on a capturing stream
-    // only marks a dependency point and is not replayed by the graph.
-    BF_TRY(ragged_enqueue(p->layout, dev, p->mixed, x, y, p0, params, status, iterations, chi2,
-                          p->uncertainties ? cov : nullptr, se, us, p->stream, nullptr),
+    // Events inside the capture only mark the fork and join points of the
+    // branches (an event recorded on a capturing stream is not replayed).
+    const RaggedIO io{x, y, p0, params, status, iterations, chi2,
+                      p->uncertainties ? cov : nullptr, se, us};
+    const RaggedFork fork{bf_ragged_plan_s::kBranches, p->side, p->fork, p->join};
+    BF_TRY(ragged_enqueue(p->layout, dev, p->mixed, io, p->stream, &fork),
            "ragged batch enqueue");
     return BF_OK;
 }
@@ -299,6 +376,12 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     for (hipEvent_t &ev : p->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess)
             return bad(e, "hipEventCreate");
+    if ((e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess)
+        return bad(e, "hipEventCreate");
+    for (int b = 0; b < bf_ragged_plan_s::kBranches; ++b)
+        if ((e = hipStreamCreateWithFlags(&p->side[b], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming)) != hipSuccess)
+            return bad(e, "branch stream");
     if ((e = ragged_prepare(p->layout, p->mixed)) != hipSuccess)
         return bad(e, "loading the ragged kernels");
     *plan = p;

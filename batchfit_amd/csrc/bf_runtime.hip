@@ -226,7 +226,7 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     const float *init = p0;
     if (!init) {
         const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-        const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
+        const uint32_t sort_n = guess_sort_floats(count);
         const size_t lds = size_t(m + sort_n) * sizeof(float);
         if (lds > 65536)
             return fail(err, errlen, BF_INVALID_ARGUMENT,
@@ -434,7 +434,7 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     hipStream_t s = rt.stream;
     const size_t P = size_t(kParams[model]);
     const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-    const uint32_t sort_n = count <= 128 ? 0 : uint32_t(pow2_ceil(2 * count));
+    const uint32_t sort_n = guess_sort_floats(count);
     const size_t lds = size_t(m + sort_n) * sizeof(float);
     if (lds > 65536)
         return fail(err, errlen, BF_INVALID_ARGUMENT,

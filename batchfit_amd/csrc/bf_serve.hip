@@ -100,8 +100,11 @@ struct PinnedArena {
 constexpr size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
 
 struct Slot {
+    static constexpr int kBranches = 4;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipStream_t side[kBranches] = {};  // branches of a batch's runs
+    hipEvent_t fork = nullptr, join[kBranches] = {};
     DeviceArena dev;
     PinnedArena host;
     std::vector<Request *> batch;
@@ -143,7 +146,9 @@ namespace {
 
 uint64_t request_bytes(const Request &r) {
     const uint64_t n = r.n, m = r.m;
-    uint64_t b = 4 * (m + n * m) + 4 * n * (8 + 8 + 3) + 32 * n + 160;
+    // Host requests are staged into the arena; device requests are read in
+    // place, so only their tables and outputs count.
+    uint64_t b = (r.device ? 0 : 4 * (m + n * m)) + 4 * n * (8 + 8 + 3) + 32 * n + 160;
     if (r.p0)
         b += 32 * n;
     if (r.cov)
@@ -158,12 +163,16 @@ void Server::launch(Slot &slot) {
     auto &batch = slot.batch;
     uint64_t xs = 0, ys = 0, p0s = 0, rows = 0;
     bool unc = false;
+    bool any_host = false;
     for (Request *r : batch) {
-        r->x_off = xs;
-        r->y_off = ys;
         r->out_row = rows;
-        xs += r->m;
-        ys += uint64_t(r->n) * r->m;
+        if (!r->device) {
+            r->x_off = xs;
+            r->y_off = ys;
+            xs += r->m;
+            ys += uint64_t(r->n) * r->m;
+            any_host = true;
+        }
         if (r->p0) {
             r->p0_row = p0s;
             p0s += r->n;
@@ -218,17 +227,29 @@ void Server::launch(Slot &slot) {
         return failed(e, "hipHostMalloc(batch staging)");
     char *H = slot.host.ptr, *D = slot.dev.ptr;
     hipStream_t s = slot.stream;
-    // Descriptors.
+    // Device requests are read in place: x and y offsets of every request
+    // are taken from the lowest address among the staged region and the
+    // device arrays (all float arrays, so every difference is whole floats).
+    uintptr_t xbase = any_host ? uintptr_t(D + d_x) : UINTPTR_MAX;
+    uintptr_t ybase = any_host ? uintptr_t(D + d_y) : UINTPTR_MAX;
+    for (Request *r : batch)
+        if (r->device) {
+            xbase = std::min(xbase, uintptr_t(r->x));
+            ybase = std::min(ybase, uintptr_t(r->y));
+        }
     auto *fits = reinterpret_cast<bf_fit_desc *>(H + h_fits);
     auto *grids = reinterpret_cast<bf_grid_desc *>(H + h_grids);
     auto *cfgs = reinterpret_cast<bf_config *>(H + h_cfg);
     uint32_t i = 0;
     for (uint32_t q = 0; q < nreq; ++q) {
         const Request &r = *batch[q];
-        grids[q] = bf_grid_desc{r.x_off, r.m, 0};
+        const uintptr_t xa = r.device ? uintptr_t(r.x) : uintptr_t(D + d_x + r.x_off * 4);
+        const uintptr_t ya = r.device ? uintptr_t(r.y) : uintptr_t(D + d_y + r.y_off * 4);
+        const uint64_t x_off = (xa - xbase) / 4, y_off = (ya - ybase) / 4;
+        grids[q] = bf_grid_desc{x_off, r.m, 0};
         cfgs[q] = r.cfg;
         for (uint32_t k = 0; k < r.n; ++k, ++i)
-            fits[i] = bf_fit_desc{r.y_off + uint64_t(k) * r.m, r.m, q, r.model, q,
+            fits[i] = bf_fit_desc{y_off + uint64_t(k) * r.m, r.m, q, r.model, q,
                                   r.p0 ? uint32_t(r.p0_row + k) : 0xffffffffu,
                                   uint32_t(r.out_row + k)};
     }
@@ -241,8 +262,8 @@ void Server::launch(Slot &slot) {
     }
     std::copy(layout.order.begin(), layout.order.end(), reinterpret_cast<uint32_t *>(H + h_order));
     // Inputs: host requests are staged through pinned memory and sent in one
-    // copy; device requests are then copied device to device into the arena
-    // (after the bulk copy, in stream order, so it cannot overwrite them).
+    // copy (x, y and p0 rows); device requests' p0 rows are then copied into
+    // the arena (after the bulk copy, in stream order).
     for (Request *r : batch) {
         if (r->device)
             continue;
@@ -260,11 +281,6 @@ void Server::launch(Slot &slot) {
     for (Request *r : batch) {
         if (!r->device)
             continue;
-        if ((e = hipMemcpyAsync(D + d_x + r->x_off * 4, r->x, size_t(r->m) * 4,
-                                hipMemcpyDeviceToDevice, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(D + d_y + r->y_off * 4, r->y, size_t(r->n) * r->m * 4,
-                                hipMemcpyDeviceToDevice, s)) != hipSuccess)
-            return failed(e, "device staging copy");
         if (r->p0) {
             const int P = kParams[r->model];
             if ((e = hipMemcpy2DAsync(D + d_p0 + r->p0_row * 32, 32, r->p0, size_t(P) * 4,
@@ -290,12 +306,14 @@ void Server::launch(Slot &slot) {
                            reinterpret_cast<uint32_t *>(D + d_cur),
                            reinterpret_cast<float *>(D + d_guess), nreq};
     auto F = [&](size_t at) { return reinterpret_cast<float *>(D + at); };
-    if ((e = ragged_enqueue(layout, dev, mixed, F(d_x), F(d_y), F(d_p0), F(d_params),
-                            reinterpret_cast<int32_t *>(D + d_status),
-                            reinterpret_cast<uint32_t *>(D + d_iter), F(d_chi2),
-                            unc ? F(d_cov) : nullptr, unc ? F(d_se) : nullptr,
-                            unc ? reinterpret_cast<int32_t *>(D + d_us) : nullptr, s, nullptr)) !=
-        hipSuccess)
+    const RaggedIO io{reinterpret_cast<const float *>(xbase),
+                      reinterpret_cast<const float *>(ybase), F(d_p0), F(d_params),
+                      reinterpret_cast<int32_t *>(D + d_status),
+                      reinterpret_cast<uint32_t *>(D + d_iter), F(d_chi2),
+                      unc ? F(d_cov) : nullptr, unc ? F(d_se) : nullptr,
+                      unc ? reinterpret_cast<int32_t *>(D + d_us) : nullptr};
+    const RaggedFork fork{Slot::kBranches, slot.side, slot.fork, slot.join};
+    if ((e = ragged_enqueue(layout, dev, mixed, io, s, &fork)) != hipSuccess)
         return failed(e, "batch enqueue");
     // Outputs: device requests receive device-to-device copies in stream
     // order; host requests are copied back to pinned memory and scattered on
@@ -470,6 +488,27 @@ void Server::complete_loop() {
 
 using namespace bfhip;
 
+namespace {
+void destroy_slots(Server &s) {
+    for (auto &slot : s.slots) {
+        if (slot->stream)
+            (void)hipStreamSynchronize(slot->stream);
+        for (hipStream_t st : slot->side)
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
+        if (slot->stream)
+            (void)hipStreamDestroy(slot->stream);
+        for (hipEvent_t e : {slot->done, slot->fork, slot->join[0], slot->join[1], slot->join[2],
+                             slot->join[3]})
+            if (e)
+                (void)hipEventDestroy(e);
+    }
+    s.slots.clear();
+}
+}  // namespace
+
 struct bf_server_s {
     Server server;
 };
@@ -507,10 +546,17 @@ int bf_serve_create(bf_server *out, const bf_serve_options *options, char *err, 
     s.mixed = o.numerics != 0;
     for (uint32_t i = 0; i < o.slots; ++i) {
         auto slot = std::make_unique<Slot>();
-        if (hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&slot->done, hipEventDisableTiming) != hipSuccess)
-            return fail(err, errlen, BF_RUNTIME_ERROR, "could not create slot stream/event");
+        bool ok = hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&slot->done, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&slot->fork, hipEventDisableTiming) == hipSuccess;
+        for (int b = 0; ok && b < Slot::kBranches; ++b)
+            ok = hipStreamCreateWithFlags(&slot->side[b], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&slot->join[b], hipEventDisableTiming) == hipSuccess;
         s.slots.push_back(std::move(slot));
+        if (!ok) {
+            destroy_slots(s);
+            return fail(err, errlen, BF_RUNTIME_ERROR, "could not create slot streams/events");
+        }
     }
     s.scheduler = std::thread([&s] { s.schedule_loop(); });
     s.completer = std::thread([&s] { s.complete_loop(); });
@@ -536,6 +582,8 @@ int bf_serve_submit(bf_server server, uint32_t model, const bf_config *cfg, int 
     if (on_device && unc)
         return fail(err, errlen, BF_INVALID_ARGUMENT,
                     "uncertainties are served for host-array requests only");
+    if (on_device && ((uintptr_t(x) | uintptr_t(y) | uintptr_t(p0)) & 3u))
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "device arrays must be 4-byte aligned");
     if (cfg->fits && (!x || !y || !params || !status || !iterations || !chi2))
         return fail(err, errlen, BF_INVALID_ARGUMENT, "x, y and every output must be non-null");
     if (!p0 && cfg->fits && cfg->points > 13000)
@@ -642,14 +690,7 @@ int bf_serve_destroy(bf_server server) {
     s.cv_complete.notify_all();
     if (s.completer.joinable())
         s.completer.join();
-    for (auto &slot : s.slots) {
-        if (slot->stream) {
-            (void)hipStreamSynchronize(slot->stream);
-            (void)hipStreamDestroy(slot->stream);
-        }
-        if (slot->done)
-            (void)hipEventDestroy(slot->done);
-    }
+    destroy_slots(s);
     delete server;
     return BF_OK;
 }

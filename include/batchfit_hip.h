@@ -14,7 +14,8 @@
  *  - Plain pointers and sizes only; no C++ types or exceptions cross the ABI.
  *  - Every entry returns BF_OK (0) or an error code and writes a message into
  *    err[errlen] when err is not NULL. Per-fit numerical problems are status
- *    codes, never errors (DESIGN.md:52-54 of the reference).
+ *    codes, never errors, as in the reference's batch contract
+ *    (cpu/batch.hpp:23-65: one FitResult per row, whatever the row holds).
  *  - Status codes are cpu/solver.hpp:16-23; uncertainty status codes are
  *    uncertainty_types.hpp:3-10; both int32. Iterations are uint32.
  *  - y is fit-major (row i = fit i, M contiguous floats) unless stated.

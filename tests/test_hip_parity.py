@@ -109,6 +109,29 @@ def test_guess_bitwise_sets(hip, oracle_port, model, m, n):
                               np.ascontiguousarray(b).view(np.uint8)), model
 
 
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("m", [1500, 2600, 4096, 6000])
+def test_guess_bitwise_long_windows(hip, oracle_port, model, m):
+    """Edge windows of 150–600 points: medians by register selection (up to
+    1024 values) and by the LDS sort beyond. Bitwise on noisy data; on a
+    quantised copy, where many equal values (zeros of both signs among them)
+    meet at the median, equal in value: which of two equal elements the
+    reference's nth_element leaves in the middle is unspecified, so the sign
+    of a zero median is not part of the contract."""
+    x, y, _, _, _ = datasets.comparison_set(model, 48, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    got = device_guess(x, y, model)
+    ref = oracle_port.guess(x, y, model, threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8)), (model, m)
+    q = (np.round(y * 50) / 50).astype(np.float32)
+    got = device_guess(x, q, model)
+    ref = oracle_port.guess(x, q, model, threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
+
+
 def test_guess_bitwise_metric_and_doublets(hip, oracle_port):
     x, y, _, _ = datasets.config2(n=1 << 20, rows=4096)
     got = device_guess(x, y, "pseudo_voigt")
