@@ -197,6 +197,111 @@ template <int STEP> __device__ __forceinline__ int xlane(int v) {
         return perm_xor32(v);
 }
 
+// Exact xor-D partner of a lane (D = 1 … 32): DPP quad_perm for 1 and 2,
+// ds_swizzle (bit mode, within 32 lanes) for 4, 8 and 16, ds_bpermute for 32.
+template <int D> __device__ __forceinline__ int xor_lane(int v) {
+    if constexpr (D == 1)
+        return dpp_quad_xor1(v);
+    else if constexpr (D == 2)
+        return dpp_quad_xor2(v);
+    else if constexpr (D == 4)
+        return __builtin_amdgcn_ds_swizzle(v, 0x101F);
+    else if constexpr (D == 8)
+        return __builtin_amdgcn_ds_swizzle(v, 0x201F);
+    else if constexpr (D == 16)
+        return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    else
+        return __shfl_xor(v, D, 64);
+}
+template <int D> __device__ __forceinline__ double xor_lane_d(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = xor_lane<D>(int(bits & 0xffffffffll));
+    const int hi = xor_lane<D>(int(bits >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// All-reduce of T doubles over an aligned group of G lanes by halving
+// (reduce-scatter): at distance D = 1, 2, 4, … each lane keeps one half of its
+// current terms — the lower half if its D bit is clear — and adds the
+// partner's copy of that half, so the data exchanged halves at every step
+// (for 15 terms over 64 lanes: 17 doubles move instead of 90). When one term
+// is left the remaining distances sum it whole. Every total then sits in a
+// known lane and is broadcast from there, so all lanes of the group hold the
+// same bits.
+namespace detail {
+struct ScatterPlan {  // where term t ends up: lane bits and slot
+    int lane, slot;
+};
+constexpr ScatterPlan scatter_place(int T, int G, int t) {
+    int count = T, slot = t, lane = 0;
+    for (int d = 1; d < G && count > 1; d <<= 1) {
+        const int keep = (count + 1) / 2;  // split point of every lane at this step
+        if (slot >= keep) {
+            lane |= d;
+            slot -= keep;
+        }
+        count = keep;
+    }
+    return {lane, slot};
+}
+template <int G, int D, int C, int T>
+__device__ __forceinline__ void scatter_steps(double (&v)[T], int gl) {
+    if constexpr (D < G) {
+        if constexpr (C > 1) {
+            constexpr int K = (C + 1) / 2;
+            const bool upper = (gl & D) != 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const double hi_half = (K + i < C) ? v[K + i] : 0.0;
+                const double keep = upper ? hi_half : v[i];
+                const double send = upper ? v[i] : hi_half;
+                v[i] = keep + xor_lane_d<D>(send);
+            }
+            scatter_steps<G, D * 2, K, T>(v, gl);
+        } else {
+            v[0] += xor_lane_d<D>(v[0]);
+            scatter_steps<G, D * 2, 1, T>(v, gl);
+        }
+    }
+    // D == G: every slot a lane still holds is a full group total.
+}
+template <int G, int T> constexpr int scatter_slots() {
+    int count = T;
+    for (int d = 1; d < G && count > 1; d <<= 1)
+        count = (count + 1) / 2;
+    return count;
+}
+}  // namespace detail
+
+template <int G, int T>
+__device__ __forceinline__ void group_allreduce_d(double (&v)[T], int gl) {
+    if constexpr (G == 1)
+        return;
+    detail::scatter_steps<G, 1, T, T>(v, gl);
+    constexpr int S = detail::scatter_slots<G, T>();
+    double part[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+        part[i] = v[i];
+    const int base = int(threadIdx.x & 63) & ~(G - 1);
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const detail::ScatterPlan pl = detail::scatter_place(T, G, t);
+        const long long bits = __double_as_longlong(part[pl.slot]);
+        int lo, hi;
+        if constexpr (G == 64) {
+            lo = __builtin_amdgcn_readlane(int(bits & 0xffffffffll), pl.lane);
+            hi = __builtin_amdgcn_readlane(int(bits >> 32), pl.lane);
+        } else {
+            lo = __shfl(int(bits & 0xffffffffll), base + pl.lane, 64);
+            hi = __shfl(int(bits >> 32), base + pl.lane, 64);
+        }
+        v[t] = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                (unsigned long long)(unsigned)lo));
+    }
+}
+
 template <int STEP> __device__ __forceinline__ float xlane_f(float v) {
     return __int_as_float(xlane<STEP>(__float_as_int(v)));
 }

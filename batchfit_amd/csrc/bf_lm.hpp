@@ -301,11 +301,27 @@ __device__ __forceinline__ void fused_pass(const Points<G, R> &pts, const float 
             acc[t].flush();
     });
     lin_ok = true;
+    if constexpr (sizeof(S) == 8) {
+        // Each lane's (sum, carry) pair as one double, then one halving
+        // all-reduce over the group for all terms together.
+        double v[T];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        terms[t] = reduce_term<G, S, COMP>(acc[t]);
-        if (t < TRI + P)
-            lin_ok = lin_ok && finite_s(terms[t]);
+        for (int t = 0; t < T; ++t)
+            v[t] = double(acc[t].hi()) - double(acc[t].lo());
+        group_allreduce_d<G, T>(v, pts.gl);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            terms[t] = v[t];
+            if (t < TRI + P)
+                lin_ok = lin_ok && finite_s(terms[t]);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            terms[t] = reduce_term<G, S, COMP>(acc[t]);
+            if (t < TRI + P)
+                lin_ok = lin_ok && finite_s(terms[t]);
+        }
     }
 }
 
@@ -341,9 +357,20 @@ __device__ __forceinline__ void information_pass(const Points<G, R> &pts, const 
     if constexpr (G > 1)
         bad = group_or<G>(bad);
     jac_ok = !bad;
+    if constexpr (sizeof(S) == 8) {
+        double v[TRI];
 #pragma unroll
-    for (int t = 0; t < TRI; ++t)
-        info[t] = reduce_term<G, S, true>(acc[t]);
+        for (int t = 0; t < TRI; ++t)
+            v[t] = double(acc[t].hi()) - double(acc[t].lo());
+        group_allreduce_d<G, TRI>(v, pts.gl);
+#pragma unroll
+        for (int t = 0; t < TRI; ++t)
+            info[t] = v[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < TRI; ++t)
+            info[t] = reduce_term<G, S, true>(acc[t]);
+    }
 }
 
 template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const float *p,
