@@ -389,4 +389,17 @@ template <int G> __device__ __forceinline__ float group_min_f(float v) {
     return v;
 }
 
+// Grid facts shared by all fits, derived on the device from x
+// (cpu/solver.hpp:58-69 grid_spacing; batch.hpp:236-240 varied_x;
+//  initial_guess.hpp:131-153 guess preconditions and edge abscissae).
+struct GridInfo {
+    float min_spacing;      // grid_spacing(): min |Δx| if x is monotone, else 0
+    int32_t varied_x;       // any x[j] != x[0]
+    int32_t x_finite;       // all x finite
+    int32_t guess_grid_ok;  // strictly increasing, finite, finite span
+    float guess_spacing;    // min (x[j] - x[j-1])
+    float left_x, right_x;  // midpoints of the two edge windows
+    int32_t count;          // edge window length max(1, M/10)
+};
+
 }  // namespace bfhip

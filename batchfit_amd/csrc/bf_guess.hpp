@@ -1,15 +1,16 @@
 // bf_guess.hpp — data-only initial guess on the device, one 64-lane wave per
 // fit (cpu/initial_guess.hpp:120-249).
 //
-// The fit's signal is staged in LDS; edge medians come
-// from bitonic networks held in registers; every scan of the reference (smoothed-maximum search, half-height
+// The fit's signal is staged in LDS; the edge medians are exact order
+// statistics selected in registers (an LDS sort for windows beyond 1024
+// values); every scan of the reference (smoothed-maximum search, half-height
 // walks, prominence walks, valley search) becomes a wave-wide reduction with
 // the reference's tie rule (first index wins). The O(1) arithmetic between
 // scans runs redundantly on all lanes with FP contraction disabled, so the
 // FP32 result equals the FP32 reference guess bit for bit.
 #pragma once
 
-#include "bf_lm.hpp"
+#include "bf_device.hpp"
 
 namespace bfhip {
 
@@ -63,7 +64,15 @@ __device__ __forceinline__ float std_lerp(float a, float b, float t) {
     return (t > 1) == (b > a) ? (b < v ? v : b) : (b > v ? v : b);
 }
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // one wave per block
+// Hand-over of LDS data between the lanes of one wave (a wave owns its LDS
+// slice; the LDS unit runs one wave's instructions in order, and the fences
+// keep the compiler from moving accesses across). Correct whether the block
+// holds one wave or several, each with its own slice.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ int wave_max_i(int v) {
     for (int o = 32; o > 0; o >>= 1)
@@ -135,133 +144,6 @@ __device__ __forceinline__ int pow2_at_least(int v) {
     return n;
 }
 
-// Ascending bitonic sort of SLOTS·64 values held in registers, element
-// e = q·64 + lane in v[q]: partners within a slot are exchanged across lanes by
-// shuffle, partners across slots stay in the lane. Fully unrolled, no LDS.
-template <int SLOTS> __device__ __forceinline__ void reg_sort(float (&v)[SLOTS], int lane) {
-    constexpr int n = SLOTS * 64;
-#pragma unroll
-    for (int k = 2; k <= n; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-#pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    const int qp = q ^ (j >> 6);
-                    if (qp > q) {
-                        const bool up = ((q * 64) & k) == 0;  // e & k, lane bits < 64 ≤ k
-                        const float a = v[q], b = v[qp];
-                        const bool swap = up ? (b < a) : (a < b);
-                        v[q] = swap ? b : a;
-                        v[qp] = swap ? a : b;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < SLOTS; ++q) {
-                    const float other = __shfl_xor(v[q], j, 64);
-                    const bool up = (((q * 64) + lane) & k) == 0;
-                    const bool lower = (lane & j) == 0;
-                    const bool keep_min = lower == up;
-                    const float lo = other < v[q] ? other : v[q];
-                    const float hi = other < v[q] ? v[q] : other;
-                    v[q] = keep_min ? lo : hi;
-                }
-            }
-        }
-    }
-}
-
-// Exact xor-J lane partner: DPP quad_perm for 1 and 2, ds_swizzle for 4, 8
-// and 16 (within 32 lanes), ds_bpermute for 32.
-template <int J> __device__ __forceinline__ float xor_partner(float v) {
-    const int b = __float_as_int(v);
-    if constexpr (J == 1)
-        return __int_as_float(dpp_quad_xor1(b));
-    else if constexpr (J == 2)
-        return __int_as_float(dpp_quad_xor2(b));
-    else if constexpr (J == 4)
-        return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x101F));
-    else if constexpr (J == 8)
-        return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x201F));
-    else if constexpr (J == 16)
-        return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x401F));
-    else
-        return __shfl_xor(v, J, 64);
-}
-
-// One compare-exchange stage (block size k, distance j) of the bitonic
-// network on register-held elements e = q·64 + lane.
-template <int SLOTS, int K, int J>
-__device__ __forceinline__ void bitonic_stage(float (&v)[SLOTS], int lane) {
-    if constexpr (J >= 64) {
-#pragma unroll
-        for (int q = 0; q < SLOTS; ++q) {
-            constexpr int js = J >> 6;
-            const int qp = q ^ js;
-            if (qp > q) {
-                const bool up = ((q * 64) & K) == 0;
-                const float a = v[q], b = v[qp];
-                const bool swap = up ? (b < a) : (a < b);
-                v[q] = swap ? b : a;
-                v[qp] = swap ? a : b;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < SLOTS; ++q) {
-            const float other = xor_partner<J>(v[q]);
-            const bool up = (((q * 64) + lane) & K) == 0;
-            const bool keep_min = ((lane & J) == 0) == up;
-            const float lo = other < v[q] ? other : v[q];
-            const float hi = other < v[q] ? v[q] : other;
-            v[q] = keep_min ? lo : hi;
-        }
-    }
-}
-
-// All stages with distance J, J/2, …, 1 of block size K.
-template <int SLOTS, int K, int J> __device__ __forceinline__ void bitonic_merge(float (&v)[SLOTS],
-                                                                                int lane) {
-    bitonic_stage<SLOTS, K, J>(v, lane);
-    if constexpr (J > 1)
-        bitonic_merge<SLOTS, K, J / 2>(v, lane);
-}
-template <int SLOTS, int K> __device__ __forceinline__ void bitonic_sort_to(float (&v)[SLOTS],
-                                                                            int lane) {
-    if constexpr (K > 2)
-        bitonic_sort_to<SLOTS, K / 2>(v, lane);
-    bitonic_merge<SLOTS, K, K / 2>(v, lane);
-}
-
-// Element k (wave-uniform) of the register-held sequence e = q·64 + lane.
-// Every slot is read with readlane first and the slot is chosen among the
-// scalar results: a select among the slots themselves is turned into an
-// indexed load by the optimiser, which moves the whole array to scratch.
-template <int SLOTS> __device__ __forceinline__ float reg_pick(const float (&v)[SLOTS], int k) {
-    const int q = k >> 6, lane = k & 63;
-    int w = __builtin_amdgcn_readlane(__float_as_int(v[0]), lane);
-#pragma unroll
-    for (int i = 1; i < SLOTS; ++i) {
-        const int r = __builtin_amdgcn_readlane(__float_as_int(v[i]), lane);
-        w = q == i ? r : w;
-    }
-    return __int_as_float(w);
-}
-
-template <int SLOTS, typename F>
-__device__ __forceinline__ float reg_median(int count, int lane, F &&get) {
-    float v[SLOTS];
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        const int e = q * 64 + lane;
-        v[q] = e < count ? get(e) : __uint_as_float(0x7f800000u);
-    }
-    reg_sort<SLOTS>(v, lane);
-    return count % 2 ? reg_pick<SLOTS>(v, count / 2)
-                     : std_midpoint(reg_pick<SLOTS>(v, count / 2 - 1), reg_pick<SLOTS>(v, count / 2));
-}
-
 // Order-preserving 32-bit key of a finite float (-0 sorts just below +0) and
 // its inverse.
 __device__ __forceinline__ uint32_t float_key(float f) {
@@ -318,16 +200,16 @@ __device__ float select_median(int count, int lane, F &&get) {
     return std_midpoint(key_float(lower), key_float(t));
 }
 
-// Median of get(0..count) (initial_guess.hpp:25-30): register sorts up to 256
-// values, exact selection up to 1024, an LDS sort in buf beyond.
+// Median of get(0..count) (initial_guess.hpp:25-30): exact selection in
+// registers up to 1024 values, an LDS sort in buf beyond.
 template <typename F>
 __device__ float wave_median(float *buf, int count, int lane, F &&get) {
     if (count <= 64)
-        return reg_median<1>(count, lane, get);
+        return select_median<1>(count, lane, get);
     if (count <= 128)
-        return reg_median<2>(count, lane, get);
+        return select_median<2>(count, lane, get);
     if (count <= 256)
-        return reg_median<4>(count, lane, get);
+        return select_median<4>(count, lane, get);
     if (count <= 512)
         return select_median<8>(count, lane, get);
     if (count <= 1024)
@@ -342,49 +224,6 @@ __device__ float wave_median(float *buf, int count, int lane, F &&get) {
     return r;
 }
 
-
-// Edge statistics for count ≤ 128 with one 256-element network
-// (initial_guess.hpp:142-169): the left window fills elements 0..127 and the
-// right window 128..255, each padded with +inf. Sorting blocks of 128 leaves
-// the left block ascending and the right block descending, which gives both
-// window medians; one more merge gives the median of the union. The absolute
-// deviations of each sorted block from its median are V-shaped (bitonic), so
-// two merges sort them for the noise median.
-struct EdgeStats {
-    float left_level, right_level, baseline, spread;
-};
-template <typename L, typename Rt>
-__device__ EdgeStats edge_stats_128(int count, int lane, L &&left_at, Rt &&right_at,
-                                    bool paired_levels) {
-    (void)paired_levels;
-    const float inf = __uint_as_float(0x7f800000u);
-    float v[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int e = (q & 1) * 64 + lane;
-        v[q] = e < count ? (q < 2 ? left_at(e) : right_at(e)) : inf;
-    }
-    bitonic_sort_to<4, 128>(v, lane);
-    EdgeStats st;
-    // Left block ascending: value of rank a sits at element a.
-    // Right block descending: value of rank a sits at element 255 − a.
-    auto left_rank = [&](int a) { return reg_pick<4>(v, a); };
-    auto right_rank = [&](int a) { return reg_pick<4>(v, 255 - a); };
-    st.left_level = count % 2 ? left_rank(count / 2)
-                              : std_midpoint(left_rank(count / 2 - 1), left_rank(count / 2));
-    st.right_level = count % 2 ? right_rank(count / 2)
-                               : std_midpoint(right_rank(count / 2 - 1), right_rank(count / 2));
-    float d[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        d[q] = fabsf(v[q] - (q < 2 ? st.left_level : st.right_level));
-    bitonic_merge<4, 256, 128>(v, lane);
-    st.baseline = std_midpoint(reg_pick<4>(v, count - 1), reg_pick<4>(v, count));
-    bitonic_merge<4, 128, 64>(d, lane);
-    bitonic_merge<4, 256, 128>(d, lane);
-    st.spread = std_midpoint(reg_pick<4>(d, count - 1), reg_pick<4>(d, count));
-    return st;
-}
 
 struct GuessPeak {
     float amplitude, center, width, left_radius, right_radius;
@@ -498,73 +337,52 @@ __device__ bool prominent_dev(const float *s, int m, int j, float h, float t) {
 }
 
 // The guess for fit `fit` of `a`, run by one whole wave.
-template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit) {
+// Result of one fit's guess, identical in every lane of the wave.
+struct GuessResult {
+    float p[8];
+    float baseline;
+    int32_t status;
+};
+
+// The guess of one fit by one whole wave (lane = lane in the wave), on the
+// wave's own LDS: s[0..m) holds the fit's raw observations on entry (it is
+// turned into the baseline-removed signal), buf has the sort floats the host
+// reserved (guess_sort_floats), y_bad says whether any observation is
+// non-finite (initial_guess.hpp:131-139).
+template <int MODEL>
+__device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, GridInfo grid,
+                                  bool y_bad, int lane) {
     constexpr int P = Params<MODEL>::P;
     constexpr bool paired = MODEL == TWO_LORENTZIAN;
-    extern __shared__ float lds[];
-    const int lane = threadIdx.x;
-    const int m = int(a.m);
-    float *s = lds;
-    float *buf = lds + a.m_pad;
-    const float *x = a.x;
-    const float *y = a.y + fit * a.ys_fit;
-    const GridInfo grid = *a.grid;
-
-    float out[8];
+    GuessResult res;
+    float *const out = res.p;
     for (int k = 0; k < 8; ++k)
         out[k] = qnan_f();
-    float baseline = qnan_f();
-    int32_t gstatus = GS_INVALID;
-    auto finish = [&]() {
-        if (lane == 0) {
-            for (int k = 0; k < P; ++k)
-                a.params[size_t(fit) * P + k] = out[k];
-            if (a.baseline)
-                a.baseline[fit] = baseline;
-            if (a.gstatus)
-                a.gstatus[fit] = gstatus;
-        }
-    };
-    if (m < 5 || !grid.guess_grid_ok)
-        return finish();
-    int bad = 0;
-    for (int j = lane; j < m; j += 64) {
-        const float v = y[j * a.ys_pt];
-        bad |= !finite_f(v);
-        s[j] = v;
-    }
-    bad = wave_or_i(bad);
-    wave_sync();
-    if (bad)
+    float &baseline = res.baseline;
+    int32_t &gstatus = res.status;
+    baseline = qnan_f();
+    gstatus = GS_INVALID;
+    auto finish = [&]() { return res; };
+    if (m < 5 || !grid.guess_grid_ok || y_bad)
         return finish();
     const int count = grid.count;
-    float left_level, right_level, spread;
-    if (count <= 128) {
-        const EdgeStats st = edge_stats_128(
-            count, lane, [&](int i) { return s[i]; }, [&](int i) { return s[m - count + i]; },
-            paired);
-        left_level = st.left_level;
-        right_level = st.right_level;
-        baseline = st.baseline;
-        spread = st.spread;
-    } else {
-        left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
-        right_level = wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
-        baseline = wave_median(buf, 2 * count, lane, [&](int i) {
-            return i < count ? s[i] : s[m - count + (i - count)];
-        });
-        spread = wave_median(buf, 2 * count, lane, [&](int i) {
-            return i < count ? fabsf(s[i] - left_level)
-                             : fabsf(s[m - count + (i - count)] - right_level);
-        });
-    }
+    // Edge levels, baseline and noise scale (initial_guess.hpp:142-169).
+    const float left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
+    const float right_level =
+        wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
+    baseline = wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? s[i] : s[m - count + (i - count)];
+    });
+    const float spread = wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? fabsf(s[i] - left_level) : fabsf(s[m - count + (i - count)] - right_level);
+    });
     const float left_x = grid.left_x, right_x = grid.right_x;
     const float slope = paired ? (right_level - left_level) / (right_x - left_x) : 0.0f;
     const float level = paired ? left_level : baseline;
     if (!finite_f(slope))
         return finish();
     const float noise = 1.4826f * spread;
-    bad = 0;
+    int bad = 0;
     for (int j = lane; j < m; j += 64) {
         const float v = s[j] - (level + slope * (x[j] - left_x));
         bad |= !finite_f(v);
@@ -670,7 +488,34 @@ template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit)
         return finish();
     }
     gstatus = GS_SUCCESS;
-    finish();
+    return finish();
+}
+
+// The guess of fit `fit` of `a`, run by one whole wave that owns the dynamic
+// LDS of its block (a.m_pad + a.sort_n floats).
+template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit) {
+    constexpr int P = Params<MODEL>::P;
+    extern __shared__ float lds[];
+    const int lane = int(threadIdx.x & 63);
+    const int m = int(a.m);
+    const float *y = a.y + fit * a.ys_fit;
+    int bad = 0;
+    for (int j = lane; j < m; j += 64) {
+        const float v = y[j * a.ys_pt];
+        bad |= !finite_f(v);
+        lds[j] = v;
+    }
+    bad = wave_or_i(bad);
+    wave_sync();
+    const GuessResult g = guess_core<MODEL>(lds, lds + a.m_pad, a.x, m, *a.grid, bad != 0, lane);
+    if (lane == 0) {
+        for (int k = 0; k < P; ++k)
+            a.params[size_t(fit) * P + k] = g.p[k];
+        if (a.baseline)
+            a.baseline[fit] = g.baseline;
+        if (a.gstatus)
+            a.gstatus[fit] = g.status;
+    }
 }
 
 template <int MODEL> __global__ void __launch_bounds__(64) guess_kernel(GuessArgs a) {
@@ -678,6 +523,7 @@ template <int MODEL> __global__ void __launch_bounds__(64) guess_kernel(GuessArg
         guess_fit<MODEL>(a, blockIdx.x);
 }
 
-#pragma clang fp contract(on)
+// The code that follows this header keeps hipcc's default contraction.
+#pragma clang fp contract(fast)
 
 }  // namespace bfhip

@@ -12,7 +12,7 @@ namespace bfhip {
 
 namespace {
 template <int MODEL, int G, int R, bool COMP, bool MIXED>
-hipError_t go(const FitArgs &args, hipStream_t stream) {
+hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
     if constexpr (!plan_instantiated(G, R, COMP, MIXED)) {
         return hipErrorInvalidValue;
     } else {
@@ -25,20 +25,20 @@ hipError_t go(const FitArgs &args, hipStream_t stream) {
 }
 
 template <int MODEL, int G, bool COMP, bool MIXED>
-hipError_t by_tile(int r, const FitArgs &args, hipStream_t stream) {
-    switch (r) {
+hipError_t by_tile(const Plan &plan, const FitArgs &args, hipStream_t stream) {
+    switch (plan.R) {
     case 0:
-        return go<MODEL, G, 0, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 0, COMP, MIXED>(plan, args, stream);
     case 2:
-        return go<MODEL, G, 2, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 2, COMP, MIXED>(plan, args, stream);
     case 4:
-        return go<MODEL, G, 4, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 4, COMP, MIXED>(plan, args, stream);
     case 8:
-        return go<MODEL, G, 8, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 8, COMP, MIXED>(plan, args, stream);
     case 16:
-        return go<MODEL, G, 16, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 16, COMP, MIXED>(plan, args, stream);
     case 32:
-        return go<MODEL, G, 32, COMP, MIXED>(args, stream);
+        return go<MODEL, G, 32, COMP, MIXED>(plan, args, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -48,10 +48,10 @@ template <> hipError_t launch_lm_group<BF_MODEL, BF_GROUP>(const Plan &plan, con
                                                            hipStream_t stream) {
     constexpr int MODEL = BF_MODEL, G = BF_GROUP;
     if (plan.mixed)
-        return plan.comp ? by_tile<MODEL, G, true, true>(plan.R, args, stream)
-                         : by_tile<MODEL, G, false, true>(plan.R, args, stream);
-    return plan.comp ? by_tile<MODEL, G, true, false>(plan.R, args, stream)
-                     : by_tile<MODEL, G, false, false>(plan.R, args, stream);
+        return plan.comp ? by_tile<MODEL, G, true, true>(plan, args, stream)
+                         : by_tile<MODEL, G, false, true>(plan, args, stream);
+    return plan.comp ? by_tile<MODEL, G, true, false>(plan, args, stream)
+                     : by_tile<MODEL, G, false, false>(plan, args, stream);
 }
 
 }  // namespace bfhip

@@ -4,7 +4,7 @@
 // launch_guess / launch_model_eval.
 #pragma once
 
-#include "bf_guess.hpp"
+#include "bf_lm.hpp"
 
 namespace bfhip {
 

@@ -25,22 +25,9 @@
 // Model value, residual and Jacobian are always FP32.
 #pragma once
 
-#include "bf_device.hpp"
+#include "bf_guess.hpp"
 
 namespace bfhip {
-
-// Grid facts shared by all fits, derived on the device from x
-// (cpu/solver.hpp:58-69 grid_spacing; batch.hpp:236-240 varied_x;
-//  initial_guess.hpp:131-153 guess preconditions and edge abscissae).
-struct GridInfo {
-    float min_spacing;      // grid_spacing(): min |Δx| if x is monotone, else 0
-    int32_t varied_x;       // any x[j] != x[0]
-    int32_t x_finite;       // all x finite
-    int32_t guess_grid_ok;  // strictly increasing, finite, finite span
-    float guess_spacing;    // min (x[j] - x[j-1])
-    float left_x, right_x;  // midpoints of the two edge windows
-    int32_t count;          // edge window length max(1, M/10)
-};
 
 struct FitArgs {
     const float *x;
@@ -535,12 +522,13 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
     pts.load(xgrid, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
 
     float init[P], p[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        init[k] = a.p0[size_t(fit) * P + k];
     bool bad = !a.options_ok || !grid.x_finite;
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        init[k] = a.p0[size_t(fit) * P + k];
+    for (int k = 0; k < P; ++k)
         bad = bad || !finite_f(init[k]);
-    }
     // Per-fit observation checks: finiteness and flatness (lm.metal:133-139).
     const float y0 = a.y[fit * a.ys_fit];
     int flags = 0;
