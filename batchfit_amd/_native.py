@@ -26,7 +26,7 @@ EXPORTS = (
     "bf_hip_last_timing", "bf_hip_malloc", "bf_hip_free", "bf_hip_memcpy", "bf_hip_synchronize",
     "bf_hip_ragged_create", "bf_hip_ragged_run", "bf_hip_ragged_timing", "bf_hip_ragged_destroy",
     "bf_hip_synthesize", "bf_serve_create", "bf_serve_submit", "bf_serve_wait", "bf_serve_stats",
-    "bf_serve_destroy",
+    "bf_serve_destroy", "bf_npy_inspect", "bf_npy_chunk_fits", "bf_hip_fit_npy",
 )
 
 

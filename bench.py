@@ -474,11 +474,11 @@ def bench_serving(S, args):
 
 
 def bench_cube(S, args):
-    """512×512×1024 pseudo-Voigt cube: generated in HBM, written to an NPY file,
-    then streamed from the file (memory map) through the server in chunks of
-    16,384 spectra (host payloads: disk → pinned → HBM → fit → host)."""
+    """512×512×1024 pseudo-Voigt cube: generated in HBM, written to an NPY
+    file, then fitted from the file by the streaming pipeline (memory-mapped
+    chunks → pinned ring → HBM → device guess and fit → host, in order)."""
+    from batchfit_amd.npy import fit_npy
     from batchfit_amd.ragged import DeviceArray
-    from batchfit_amd.serve import Server
     n, m = 512 * 512, 1024
     x = np.linspace(-10, 10, m).astype(np.float32)
     dx, dy = DeviceArray.of(x), DeviceArray(n * m * 4)
@@ -486,8 +486,9 @@ def bench_cube(S, args):
     S.synthesize("pseudo_voigt", n, dx.ptr, m, lo, hi, dy.ptr, 0.01, True, 31, 0)
     S.sync()
     folder = tempfile.mkdtemp(prefix="bf_cube_")
-    path = os.path.join(folder, "cube_y.npy")
-    mm = np.lib.format.open_memmap(path, mode="w+", dtype=np.float32, shape=(n, m))
+    xpath, ypath = os.path.join(folder, "cube_x.npy"), os.path.join(folder, "cube_y.npy")
+    np.save(xpath, x)
+    mm = np.lib.format.open_memmap(ypath, mode="w+", dtype=np.float32, shape=(n, m))
     chunk = 1 << 14
     for r in range(0, n, chunk):
         dy.view(r * m * 4, chunk * m * 4).download(mm[r:r + chunk])
@@ -495,30 +496,23 @@ def bench_cube(S, args):
     del mm
     dx.free()
     dy.free()
-    y = np.load(path, mmap_mode="r")
     bounds = ([0, -10, 1e-4, 0], [INF, 10, 10, 1])
+    fit_npy(xpath, ypath, "pseudo_voigt", None, bounds, chunk_fits=4096)  # warm-up
     t0 = time.perf_counter()
-    lat = []
-    with Server(slots=3) as srv:
-        pending = []
-        for r in range(0, n, chunk):
-            block = np.ascontiguousarray(y[r:r + chunk])
-            pending.append(srv.submit(x, block, "pseudo_voigt", None, bounds))
-            while len(pending) > 3:
-                t = pending.pop(0)
-                t.wait(timeout=600)
-                lat.append(t.timing.latency * 1e3)
-        for t in pending:
-            t.wait(timeout=600)
-            lat.append(t.timing.latency * 1e3)
+    result, stats = fit_npy(xpath, ypath, "pseudo_voigt", None, bounds)
     elapsed = time.perf_counter() - t0
-    os.remove(path)
+    os.remove(xpath)
+    os.remove(ypath)
     os.rmdir(folder)
-    log(f"cube streamed from NPY: {n / elapsed:,.0f} fits/s ({elapsed:.2f}s for {n} spectra)")
-    return {"spectra": n, "points": m, "chunk": chunk, "fits_per_s": n / elapsed,
-            "seconds": elapsed, "chunk_latency_ms": {"p50": float(np.percentile(lat, 50)),
-                                                     "p99": float(np.percentile(lat, 99))},
-            "path": "NPY memory map (page cache) -> pinned staging -> HBM -> fit -> host"}
+    log(f"cube from NPY: {n / elapsed:,.0f} fits/s ({elapsed:.2f}s for {n} spectra, "
+        f"{stats['chunks']} chunks of {stats['chunk_fits']})")
+    return {"spectra": n, "points": m, "fits_per_s": n / elapsed, "seconds": elapsed,
+            "chunks": stats["chunks"], "chunk_fits": stats["chunk_fits"],
+            "max_in_flight": stats["max_in_flight"],
+            "prepare_seconds": stats["prepare_seconds"],
+            "status_counts": np.bincount(result.status, minlength=6).tolist(),
+            "path": "NPY file (page cache) -> memory-mapped windows -> pinned ring -> HBM -> "
+                    "device guess + fit -> host, in order (bf_hip_fit_npy)"}
 
 
 # --------------------------------------------------------------------------

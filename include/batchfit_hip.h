@@ -196,6 +196,43 @@ int bf_serve_stats(bf_server server, bf_serve_counters *stats);
 /* Drain every admitted and pending request, then stop the threads. */
 int bf_serve_destroy(bf_server server);
 
+/* NPY inputs and the streaming pipeline (metal::fit_npy / stream_npy,
+ * include/batchfit/metal/pipeline.hpp:49-67, src/metal/pipeline.cpp:70-265;
+ * NPY rules of src/cpu/npy.cpp:51-84). Inputs are NPY v1.0 files, C order,
+ * little-endian float32 or float64: x[M], y[N,M] and optionally p0[N,P].
+ * Chunks of fits are mapped from the file, staged through a ring of pinned
+ * slots, narrowed to float32 on the device when stored as float64, fitted
+ * with the device guess (or p0) on the slot's stream, and handed to
+ * `consume` in order of first_fit; a nonzero return from `consume` stops the
+ * pipeline with BF_RUNTIME_ERROR. Output pointers passed to `consume` are
+ * valid only during the call (params [fits,P], cov [fits,P,P] row-major). */
+typedef struct bf_npy_options {
+    uint64_t chunk_fits;    /* fits per chunk; 0 = from budget_bytes */
+    uint32_t in_flight;     /* chunks in flight, 1..8 (the reference allows 1..3) */
+    int32_t uncertainties;  /* also compute covariance, stderr, status */
+    uint64_t budget_bytes;  /* bytes the ring may hold per fit column; 0 = 1 GiB */
+} bf_npy_options;
+typedef struct bf_npy_stats {
+    uint64_t fits, chunks, chunk_fits, max_in_flight, max_mapped_bytes, shared_payload_bytes;
+    double wall_seconds, prepare_seconds, device_seconds, readback_seconds;
+} bf_npy_stats;
+typedef int (*bf_npy_consumer)(void *user, uint64_t first_fit, uint64_t fits, const float *params,
+                               const int32_t *status, const uint32_t *iterations, const float *chi2,
+                               const float *cov, const float *stderr_, const int32_t *ustatus);
+/* Header of an NPY file: shape[2] (shape[1] = 0 for rank 1), rank, dtype
+ * (0 float32, 1 float64, 2 uint32), payload offset (npy::inspect,
+ * include/batchfit/npy.hpp:19). Needs no device. */
+int bf_npy_inspect(const char *path, uint64_t *shape, uint32_t *rank, uint32_t *dtype,
+                   uint64_t *offset, char *err, size_t errlen);
+/* Fits per chunk for these dimensions (choose_chunk_fits,
+ * src/metal/pipeline.cpp:28-68): chunk_fits when set, else the budget over
+ * the ring's bytes per fit, rounded down to a multiple of 256. 0 if invalid. */
+uint64_t bf_npy_chunk_fits(uint64_t fits, uint64_t points, uint32_t parameters,
+                           uint32_t input_bytes, uint32_t p0_bytes, const bf_npy_options *options);
+int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const char *p0_path,
+                   const bf_config *cfg, const bf_npy_options *options, bf_npy_consumer consume,
+                   void *user, bf_npy_stats *stats, char *err, size_t errlen);
+
 /* Seeded synthetic spectra written straight into HBM (benchmarks, serving
  * load): y[n*m] (fit-major) of `model` on the device grid d_x[m], parameters
  * per fit drawn uniformly from [lower[k], upper[k]) (8 entries), noise

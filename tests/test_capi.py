@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "batchfit_hip.h")).read()
-    return sorted(set(re.findall(r"\b(bf_(?:hip|serve)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(bf_(?:hip|serve|npy)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
