@@ -1,0 +1,194 @@
+"""Autotuner of the fit-to-hardware mapping (lanes per fit G, by model and M).
+
+    python -m batchfit_amd.autotune [--fits 65536] [--repeat 3] [--write]
+
+For every model and a ladder of point counts M, fits of that shape are
+generated in HBM (Philox generator, parameters as in the benchmark
+workloads), started from their true parameters so the guess plays no part,
+and solved by the LM kernel with each candidate mapping — 16, 32 or 64 lanes
+per fit, the register tile the smallest power of two ≥ M/G (streaming beyond
+32 points per lane) — in the production numerics (compensated partials, FP64
+solve). The kernel time is read from the library's events; the fastest
+mapping wins each (model, M). Adjacent winners are merged into ranges and,
+with --write, emitted as batchfit_amd/csrc/bf_tuned.hpp (the table both the
+uniform and the ragged paths use), followed by a rebuild. The measured table
+is printed as JSON.
+
+The reference picks among its kernel variants by a measured policy for its
+Apple GPU (include/batchfit/metal/batch.hpp:21-50); this is the same idea for
+gfx950, measured rather than hand-written.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+from . import _native
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODELS = ("gaussian", "lorentzian", "pseudo_voigt", "two_lorentzian")
+LADDER = (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096)
+CANDIDATES = (16, 32, 64)
+RANGES = {  # parameters drawn per fit, and noise (σ relative to amplitude unless absolute)
+    "gaussian": ([0.5, -3, 0.3], [5, 3, 1.5]),
+    "lorentzian": ([0.5, -3, 0.3], [5, 3, 1.5]),
+    "pseudo_voigt": ([0.5, -3, 0.6, 0.0], [5, 3, 3.0, 1.0]),
+    "two_lorentzian": ([0.5, -5, 0.3, 0.5, 0, 0, 0, 0], [5, 0, 1.0, 5, 0, 0, 0, 0]),
+}
+P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
+
+
+def tile(g: int, m: int) -> int:
+    r = -(-m // g)
+    t = 2
+    while t < r:
+        t *= 2
+    return t if t <= 32 else 0
+
+
+def measure(fits: int, repeat: int) -> dict:
+    from .ragged import DeviceArray
+    from .synth import synthesize
+    lib = _native.library()
+    if not lib.bf_hip_available():
+        raise SystemExit("no usable gfx950 device")
+    err = _native.error_buffer()
+    lib.bf_hip_set_timing(1)
+    table = {}
+    for model in MODELS:
+        p = P[model]
+        lo, hi = RANGES[model]
+        for m in LADDER:
+            x = np.linspace(-10, 10, m).astype(np.float32)
+            dx, dy, dt = DeviceArray.of(x), DeviceArray(fits * m * 4), DeviceArray(fits * 32)
+            synthesize(model, fits, dx, m, lo, hi, dy, noise=0.01, noise_relative=True, seed=41,
+                       layout="doublet" if model == "two_lorentzian" else "independent",
+                       separation=(3.0, 6.0),
+                       truth=dt)
+            truth = dt.download(np.empty((fits, 8), np.float32))[:, :p].copy()
+            dp0 = DeviceArray.of(truth)
+            outs = [DeviceArray(fits * p * 4), DeviceArray(fits * 4), DeviceArray(fits * 4),
+                    DeviceArray(fits * 4)]
+            cfg = _native.default_config()
+            cfg.fits, cfg.points, cfg.numerics = fits, m, 5
+            times = {}
+            for g in CANDIDATES:
+                r = tile(g, m)
+                if r == 0 and g != 64:
+                    continue  # only whole waves stream
+                launch = _native.Launch()
+                launch.mapping, launch.group_lanes = _native.MAP_GROUP, g
+                best = float("inf")
+                for _ in range(repeat + 1):
+                    _native.check(lib.bf_hip_fit_device(
+                        _native.MODELS[model], ctypes.byref(cfg), ctypes.byref(launch), dx.ptr,
+                        dy.ptr, dp0.ptr, *[o.ptr for o in outs], None, None, None, err,
+                        len(err)), err)
+                    t = (ctypes.c_double * 4)()
+                    lib.bf_hip_last_timing(t, 4)
+                    best = min(best, t[3])
+                times[g] = best
+            winner = min(times, key=times.get)
+            table[(model, m)] = (winner, times)
+            print(f"{model:15s} M={m:5d} " +
+                  " ".join(f"G{g}={times[g]:.3f}ms" for g in sorted(times)) + f" -> {winner}",
+                  file=sys.stderr, flush=True)
+            for a in [dx, dy, dt, dp0] + outs:
+                a.free()
+    return table
+
+
+def ranges(table: dict) -> dict:
+    """Per model: list of (largest M, lanes) covering every M. Adjacent ladder
+    points with the same winner are merged; between two winners the boundary
+    is the midpoint of their M, but never beyond what the lower winner's
+    largest register tile holds (32 points per lane)."""
+    out = {}
+    for model in MODELS:
+        pts = [(m, table[(model, m)][0]) for m in LADDER]
+        merged = []
+        for i, (m, g) in enumerate(pts):
+            if i + 1 < len(pts):
+                upper = (m + pts[i + 1][0]) // 2
+                if g < 64:
+                    upper = min(upper, 32 * g)
+            else:
+                upper = 0xFFFFFFFF if g == 64 else 32 * g
+            if merged and merged[-1][1] == g:
+                merged[-1] = (upper, g)
+            else:
+                merged.append((upper, g))
+        if merged[-1][0] != 0xFFFFFFFF:
+            merged.append((0xFFFFFFFF, 64))
+        out[model] = merged
+    return out
+
+
+def write_header(rng: dict, path: str) -> None:
+    width = max(len(v) for v in rng.values())
+    lines = [
+        "// bf_tuned.hpp — generated by `python -m batchfit_amd.autotune --write`: lanes per",
+        "// fit by (model, M), measured on gfx950 (LM kernel time of the production numerics",
+        "// from true parameters; see batchfit_amd/autotune.py). Rows: gaussian, lorentzian,",
+        "// pseudo_voigt, two_lorentzian; each entry {largest M, lanes}, the last one open.",
+        "#pragma once",
+        "",
+        "#include <stdint.h>",
+        "",
+        "namespace bfhip {",
+        "",
+        "struct TunedRange {",
+        "    uint32_t max_points;",
+        "    int lanes;",
+        "};",
+        f"constexpr int kTunedRanges = {width};",
+        "constexpr TunedRange kTuned[4][kTunedRanges] = {",
+    ]
+    for model in MODELS:
+        entries = list(rng[model])
+        while len(entries) < width:
+            entries.append(entries[-1])
+        body = ", ".join(f"{{{m if m != 0xFFFFFFFF else '0xffffffffu'}, {g}}}" for m, g in entries)
+        lines.append(f"    {{{body}}},  // {model}")
+    lines += ["};", "", "}  // namespace bfhip", ""]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m batchfit_amd.autotune")
+    ap.add_argument("--fits", type=int, default=65536)
+    ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--write", action="store_true",
+                    help="write csrc/bf_tuned.hpp and rebuild the library")
+    ap.add_argument("--from-json", help="take the measurements from a saved JSON line "
+                                        "instead of measuring")
+    args = ap.parse_args(argv)
+    if args.from_json:
+        saved = json.loads(open(args.from_json).read().strip().splitlines()[-1])
+        table = {}
+        for key, v in saved["measured"].items():
+            model, m = key.split(":")
+            table[(model, int(m))] = (v["winner"], {int(g): t for g, t in v["ms"].items()})
+        args.fits = saved["fits"]
+    else:
+        table = measure(args.fits, args.repeat)
+    rng = ranges(table)
+    print(json.dumps({"fits": args.fits,
+                      "measured": {f"{k[0]}:{k[1]}": {"winner": v[0], "ms": v[1]}
+                                   for k, v in table.items()},
+                      "ranges": rng}))
+    if args.write:
+        write_header(rng, os.path.join(HERE, "csrc", "bf_tuned.hpp"))
+        from .build import build
+        build()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "bf_lm.hpp"
+#include "bf_tuned.hpp"
 
 namespace bfhip {
 
@@ -106,12 +107,14 @@ template <int MODEL> hipError_t launch_guess_model(const GuessArgs &args, size_t
                                                    hipStream_t stream);
 template <int MODEL> hipError_t launch_eval_model(const EvalArgs &args, hipStream_t stream);
 
-// Tuned lanes per fit by (model, M): 16-lane groups for short spectra, whole
-// waves beyond (the same table drives uniform and ragged batches, so a fit's
-// arithmetic does not depend on how it was batched).
+// Lanes per fit by (model, M) from the measured table (bf_tuned.hpp). The
+// same table drives uniform and ragged batches, so a fit's arithmetic does
+// not depend on how it was batched.
 inline int tuned_group(int model, uint32_t m) {
-    (void)model;
-    return m <= 256 ? 16 : 64;
+    for (const TunedRange &r : kTuned[model & 3])
+        if (m <= r.max_points)
+            return r.lanes;
+    return 64;
 }
 
 // Instantiated plans: lane-per-fit with every numerics form; 64-lane groups

@@ -186,11 +186,15 @@ __device__ __forceinline__ S reduce_term(const Acc<COMP> &a) {
     }
 }
 
+// Points per lane loaded at once when streaming (R == 0): 16, or 8 for the
+// two-peak model, whose larger state leaves fewer registers free.
+constexpr int stream_chunk(int model) { return model == TWO_LORENTZIAN ? 8 : 16; }
+
 // Points of one fit as seen by one lane: a register tile (R > 0) or a
-// strided stream from memory (R == 0).
+// strided stream from memory (R == 0), CHUNK points per lane at a time.
 // With a register tile, x is read from the block's LDS copy of the shared
 // grid (every fit of a launch shares it), which keeps R VGPRs free.
-template <int G, int R> struct Points {
+template <int G, int R, int CHUNK = 16> struct Points {
     float yv[R > 0 ? R : 1];
     const float *x, *y;
     long long ys_pt;
@@ -214,6 +218,7 @@ template <int G, int R> struct Points {
     // of kBlock points. Slots below m/G are full for every lane of the group
     // (a scalar branch); only slot m/G is partial.
     static constexpr int kBlock = 4;
+    static constexpr int kChunk = CHUNK;
     template <typename F, typename Flush>
     __device__ __forceinline__ void each(F &&f, Flush &&flush) const {
         if constexpr (R > 0) {
@@ -240,8 +245,30 @@ template <int G, int R> struct Points {
                     flush();
             }
         } else {
+            int j = gl;
+            if constexpr (G > 1) {
+                // Whole chunks of kChunk points per lane: all their loads are
+                // issued before the first point is evaluated, so one memory
+                // latency is paid per chunk instead of per point. Points and
+                // flushes come in the same order as in the per-point loop.
+                static_assert(kChunk % kBlock == 0, "chunks hold whole blocks");
+                for (; j + G * (kChunk - 1) < m; j += G * kChunk) {
+                    float xc[kChunk], yc[kChunk];
+#pragma unroll
+                    for (int c = 0; c < kChunk; ++c) {
+                        xc[c] = x[j + G * c];
+                        yc[c] = y[(j + G * c) * ys_pt];
+                    }
+#pragma unroll
+                    for (int c = 0; c < kChunk; ++c) {
+                        f(xc[c], yc[c]);
+                        if (c % kBlock == kBlock - 1)
+                            flush();
+                    }
+                }
+            }
             int k = 0;
-            for (int j = gl; j < m; j += G) {
+            for (; j < m; j += G) {
                 f(x[j], y[j * ys_pt]);
                 if (++k == kBlock) {
                     flush();
@@ -261,7 +288,7 @@ template <int G, int R> struct Points {
 // Jᵀr (terms[TRI..TRI+P)), reduced over the group. `lin_ok` is the finiteness
 // check of the linearisation (cpu/solver.hpp:182-190).
 template <int MODEL, int G, int R, bool COMP, typename S>
-__device__ __forceinline__ void fused_pass(const Points<G, R> &pts, const float *p, S *terms,
+__device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL)> &pts, const float *p, S *terms,
                                            bool &lin_ok) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI, T = Params<MODEL>::TERMS;
     ModelConsts<MODEL> mc;
@@ -316,7 +343,7 @@ __device__ __forceinline__ void fused_pass(const Points<G, R> &pts, const float 
 // (cpu/uncertainty.hpp:297-309); `jac_ok` is false if any Jacobian entry is
 // non-finite.
 template <int MODEL, int G, int R, typename S>
-__device__ __forceinline__ void information_pass(const Points<G, R> &pts, const float *p,
+__device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk(MODEL)> &pts, const float *p,
                                                  S *info, bool &jac_ok) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     ModelConsts<MODEL> mc;
@@ -374,7 +401,7 @@ template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const fl
 // Covariance at the accepted point (cpu/uncertainty.hpp:13-110), written for
 // one fit by lane 0 of its group.
 template <int MODEL, int G, int R, typename S>
-__device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R> &pts, uint32_t fit,
+__device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, stream_chunk(MODEL)> &pts, uint32_t fit,
                                            int gl, const float *p, int32_t status, float chi2) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     const float nan = qnan_f();
@@ -518,7 +545,7 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
                                           const float *xgrid) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     const GridInfo grid = *a.grid;
-    Points<G, R> pts;
+    Points<G, R, stream_chunk(MODEL)> pts;
     pts.load(xgrid, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
 
     float init[P], p[P];

@@ -22,14 +22,13 @@ hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream
 template <int MODEL, bool MIXED>
 hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hipStream_t stream) {
     switch (cls) {
-    case 0:
-        return ragged_go<MODEL, 0, MIXED>(args, blocks, stream);
-    case 1:
-        return ragged_go<MODEL, 1, MIXED>(args, blocks, stream);
-    case 2:
-        return ragged_go<MODEL, 2, MIXED>(args, blocks, stream);
-    case 3:
-        return ragged_go<MODEL, 3, MIXED>(args, blocks, stream);
+    case 0: return ragged_go<MODEL, 0, MIXED>(args, blocks, stream);
+    case 1: return ragged_go<MODEL, 1, MIXED>(args, blocks, stream);
+    case 2: return ragged_go<MODEL, 2, MIXED>(args, blocks, stream);
+    case 3: return ragged_go<MODEL, 3, MIXED>(args, blocks, stream);
+    case 4: return ragged_go<MODEL, 4, MIXED>(args, blocks, stream);
+    case 5: return ragged_go<MODEL, 5, MIXED>(args, blocks, stream);
+    case 6: return ragged_go<MODEL, 6, MIXED>(args, blocks, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -42,18 +41,17 @@ template <int MODEL, int CLS, bool MIXED> const void *ragged_symbol() {
 // Loads the code object of a class's kernels (hipFuncGetAttributes) so that
 // the first use inside a stream capture does not have to.
 template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess) {
-    const void *sym = nullptr;
-    switch (cls * 2 + (mixed ? 1 : 0)) {
-    case 0: sym = ragged_symbol<BF_MODEL, 0, false>(); break;
-    case 1: sym = ragged_symbol<BF_MODEL, 0, true>(); break;
-    case 2: sym = ragged_symbol<BF_MODEL, 1, false>(); break;
-    case 3: sym = ragged_symbol<BF_MODEL, 1, true>(); break;
-    case 4: sym = ragged_symbol<BF_MODEL, 2, false>(); break;
-    case 5: sym = ragged_symbol<BF_MODEL, 2, true>(); break;
-    case 6: sym = ragged_symbol<BF_MODEL, 3, false>(); break;
-    case 7: sym = ragged_symbol<BF_MODEL, 3, true>(); break;
-    default: return hipErrorInvalidValue;
-    }
+    static constexpr const void *(*table[kRaggedClassCount][2])() = {
+        {&ragged_symbol<BF_MODEL, 0, false>, &ragged_symbol<BF_MODEL, 0, true>},
+        {&ragged_symbol<BF_MODEL, 1, false>, &ragged_symbol<BF_MODEL, 1, true>},
+        {&ragged_symbol<BF_MODEL, 2, false>, &ragged_symbol<BF_MODEL, 2, true>},
+        {&ragged_symbol<BF_MODEL, 3, false>, &ragged_symbol<BF_MODEL, 3, true>},
+        {&ragged_symbol<BF_MODEL, 4, false>, &ragged_symbol<BF_MODEL, 4, true>},
+        {&ragged_symbol<BF_MODEL, 5, false>, &ragged_symbol<BF_MODEL, 5, true>},
+        {&ragged_symbol<BF_MODEL, 6, false>, &ragged_symbol<BF_MODEL, 6, true>}};
+    if (cls < 0 || cls >= kRaggedClassCount)
+        return hipErrorInvalidValue;
+    const void *sym = table[cls][mixed ? 1 : 0]();
     hipFuncAttributes attr;
     hipError_t e = hipFuncGetAttributes(&attr, sym);
     if (e == hipSuccess && guess)

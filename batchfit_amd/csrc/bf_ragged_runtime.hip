@@ -90,7 +90,7 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
     for (uint32_t g = 0; g < grids; ++g)
         if (grid_table[g].points == 0)
             return "every grid needs M>0";
-    std::vector<std::vector<uint32_t>> bucket(16);
+    std::vector<std::vector<uint32_t>> bucket(4 * kRaggedClassCount);
     for (uint32_t i = 0; i < fits; ++i) {
         const bf_fit_desc &d = fit_table[i];
         if (d.model > 3 || d.grid >= grids || d.config >= configs ||
@@ -102,15 +102,16 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;
         out.any_p0 = out.any_p0 || d.p0_row != 0xffffffffu;
         out.out_rows = std::max(out.out_rows, d.out_row + 1);
-        bucket[d.model * 4 + ragged_class_of(d.points)].push_back(i);
+        bucket[d.model * kRaggedClassCount + ragged_class_of(d.model, d.points)].push_back(i);
     }
     out.order.reserve(fits);
-    for (int b = 0; b < 16; ++b) {
+    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
         auto &v = bucket[b];
         if (v.empty())
             continue;
         sort_longest_first(v, fit_table);
-        RaggedRun r{uint32_t(b / 4), b % 4, uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0};
+        RaggedRun r{uint32_t(b / kRaggedClassCount), b % kRaggedClassCount,
+                    uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0};
         for (uint32_t i : v) {
             r.guesses += fit_table[i].p0_row == 0xffffffffu;
             r.max_points = std::max(r.max_points, fit_table[i].points);
@@ -367,7 +368,8 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if ((e = hipMalloc(reinterpret_cast<void **>(&p->d_info), sizeof(GridInfo) * (grids ? grids : 1))) !=
         hipSuccess)
         return bad(e, "hipMalloc(grid info)");
-    if ((e = hipMalloc(reinterpret_cast<void **>(&p->d_cursors), sizeof(uint32_t) * 16)) != hipSuccess)
+    if ((e = hipMalloc(reinterpret_cast<void **>(&p->d_cursors),
+                       sizeof(uint32_t) * kRaggedCounters)) != hipSuccess)
         return bad(e, "hipMalloc(queue counters)");
     if (p->any_guess &&
         (e = hipMalloc(reinterpret_cast<void **>(&p->d_guess),

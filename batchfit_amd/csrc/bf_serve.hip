@@ -193,7 +193,7 @@ void Server::launch(Slot &slot) {
     const size_t d_x = carve(xs * 4), d_y = carve(ys * 4), d_p0 = carve(p0s * 32);
     const size_t d_fits = carve(rows * sizeof(bf_fit_desc)), d_grids = carve(nreq * sizeof(bf_grid_desc));
     const size_t d_cfg = carve(nreq * sizeof(bf_config)), d_info = carve(nreq * sizeof(GridInfo));
-    const size_t d_order = carve(rows * 4), d_cur = carve(16 * 4), d_guess = carve(rows * 32);
+    const size_t d_order = carve(rows * 4), d_cur = carve(kRaggedCounters * 4), d_guess = carve(rows * 32);
     const size_t d_params = carve(rows * 32), d_status = carve(rows * 4), d_iter = carve(rows * 4),
                  d_chi2 = carve(rows * 4);
     const size_t d_cov = unc ? carve(rows * 256) : 0, d_se = unc ? carve(rows * 32) : 0,

@@ -1,0 +1,66 @@
+"""The measured mapping table (csrc/bf_tuned.hpp) and the autotuner's host
+logic: ranges are increasing, end open with whole waves, and never ask a
+partial wave for more than its largest register tile (32 points per lane)."""
+import os
+import re
+
+from batchfit_amd import autotune
+
+HEADER = os.path.join(os.path.dirname(autotune.__file__), "csrc", "bf_tuned.hpp")
+
+
+def parse_table():
+    text = open(HEADER).read()
+    rows = re.findall(r"\{(\{.*\})\},\s*//\s*(\w+)", text)
+    out = {}
+    for body, model in rows:
+        pairs = re.findall(r"\{(0x[0-9a-f]+u|\d+),\s*(\d+)\}", body)
+        out[model] = [(int(m.rstrip("u"), 0), int(g)) for m, g in pairs]
+    return out
+
+
+def test_table_covers_every_model_and_m():
+    table = parse_table()
+    assert list(table) == list(autotune.MODELS)
+    for model, entries in table.items():
+        assert entries[-1] == (0xFFFFFFFF, 64), model
+        bounds = [m for m, _ in entries]
+        assert bounds == sorted(bounds), model
+        for m, g in entries:
+            assert g in (16, 32, 64), model
+            if g < 64:
+                assert m <= 32 * g, (model, m, g)
+
+
+def lookup(entries, m):
+    for bound, g in entries:
+        if m <= bound:
+            return g
+    return 64
+
+
+def test_ranges_merge_and_clamp():
+    # Synthetic measurements: 16 lanes win to 512, 32 to 1024, then 64.
+    winners = {m: 16 if m <= 512 else 32 if m <= 1024 else 64 for m in autotune.LADDER}
+    table = {(model, m): (g, {g: 1.0}) for model in autotune.MODELS for m, g in winners.items()}
+    rng = autotune.ranges(table)
+    for model in autotune.MODELS:
+        entries = rng[model]
+        for m in autotune.LADDER:
+            assert lookup(entries, m) == winners[m], (model, m)
+        # The 16-lane range ends where its largest tile does, not at the midpoint 640.
+        assert entries[0] == (512, 16)
+        assert entries[-1] == (0xFFFFFFFF, 64)
+
+
+def test_ranges_append_open_whole_wave_range():
+    table = {(model, m): (16 if m <= 512 else 32, {}) for model in autotune.MODELS
+             for m in autotune.LADDER if m <= 1024}
+    for model in autotune.MODELS:
+        for m in autotune.LADDER:
+            table.setdefault((model, m), (32, {}))
+    # 32 lanes "win" beyond their tile capacity: the table must still end in 64.
+    rng = autotune.ranges(table)
+    for model in autotune.MODELS:
+        assert rng[model][-1] == (0xFFFFFFFF, 64)
+        assert all(m <= 32 * g for m, g in rng[model] if g < 64)
