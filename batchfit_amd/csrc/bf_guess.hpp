@@ -21,7 +21,7 @@ struct GuessArgs {
     const GridInfo *grid;
     uint32_t n, m;
     uint32_t m_pad;   // LDS floats reserved for the signal
-    uint32_t sort_n;  // LDS floats reserved for sorting (guess_sort_floats)
+    uint32_t sort_n;  // LDS floats reserved beyond the signal (guess_extra_floats)
     float *params;    // [N,P]
     float *baseline;  // [N] or null
     int32_t *gstatus; // [N] or null
@@ -37,6 +37,17 @@ __host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
     while (n < 2 * count)
         n <<= 1;
     return n;
+}
+
+// Largest M the device guess takes: the signal and the sort buffer of its
+// edge medians fit in 64 KiB of LDS (12288 + 4096 floats).
+constexpr uint32_t kGuessMaxPoints = 12288;
+
+// LDS floats the guess of `model` needs beyond the signal of M points (the
+// sort buffer of its edge medians).
+__host__ __device__ inline uint32_t guess_extra_floats(int model, uint32_t m) {
+    (void)model;
+    return guess_sort_floats(m / 10 > 1 ? m / 10 : 1);
 }
 
 enum : int32_t { GS_SUCCESS = 0, GS_INVALID = 1, GS_NO_PEAK = 2, GS_INSUFFICIENT = 3 };
@@ -303,35 +314,37 @@ __device__ __forceinline__ int raw_peak_dev(const float *s, int i) {
 
 // Whether the smoothed maximum at j of height h has prominence > t
 // (initial_guess.hpp:38-54 and the `<= 3·noise` test of :212), decided by
-// one lane. The reference walks each side to the first higher point, takes
-// the side minima L and R and compares h − max(L, R) with t. Rounding of
-// h − v is monotone in v, so the test holds iff h − L > t and h − R > t, and
-// each side can stop as soon as its running minimum passes: a walk ends at
-// the first higher point, at the first point deep enough, or at the edge,
-// instead of always reaching the first higher point.
-__device__ bool prominent_dev(const float *s, int m, int j, float h, float t) {
-    float lo = h;
+// the whole wave for one candidate (j, h, t wave-uniform). The reference
+// walks each side to the first higher point, takes the side minima L and R
+// and compares h − max(L, R) with t. Rounding of h − v is monotone in v, so
+// the test holds iff h − L > t and h − R > t, and h − L > t iff some point
+// of the walk has h − v > t: a side passes iff, going outwards from j, a
+// point with h − v > t comes before the first point above h. The wave looks
+// at 64 points of a side per step (lane i at distance 1 + i from the step's
+// start); a step that holds such a point decides the side by the nearer of
+// the two kinds. t ≥ 0, so no point is of both kinds, and the points beyond
+// an edge (taken as v = h) are of neither.
+__device__ bool prominent_wave(const float *s, int m, int j, float h, float t, int lane) {
     bool left = false;
-    for (int k = j - 1; k >= 0; --k) {
-        const float v = smooth_at(s, m, k);
-        if (v > h)
-            break;
-        lo = (v < lo) ? v : lo;
-        if (h - lo > t) {
-            left = true;
+    for (int k0 = j - 1; k0 >= 0; k0 -= 64) {
+        const int k = k0 - lane;
+        const float v = k >= 0 ? smooth_at(s, m, k) : h;
+        const uint64_t higher = __ballot(v > h), deep = __ballot(h - v > t);
+        const uint64_t ends = higher | deep;
+        if (ends) {
+            left = (deep & (ends & (~ends + 1))) != 0;
             break;
         }
     }
     if (!left)
         return false;
-    lo = h;
-    for (int k = j + 1; k < m; ++k) {
-        const float v = smooth_at(s, m, k);
-        if (v > h)
-            break;
-        lo = (v < lo) ? v : lo;
-        if (h - lo > t)
-            return true;
+    for (int k0 = j + 1; k0 < m; k0 += 64) {
+        const int k = k0 + lane;
+        const float v = k < m ? smooth_at(s, m, k) : h;
+        const uint64_t higher = __ballot(v > h), deep = __ballot(h - v > t);
+        const uint64_t ends = higher | deep;
+        if (ends)
+            return (deep & (ends & (~ends + 1))) != 0;
     }
     return false;
 }
@@ -347,7 +360,7 @@ struct GuessResult {
 // The guess of one fit by one whole wave (lane = lane in the wave), on the
 // wave's own LDS: s[0..m) holds the fit's raw observations on entry (it is
 // turned into the baseline-removed signal), buf has the sort floats the host
-// reserved (guess_sort_floats), y_bad says whether any observation is
+// reserved (guess_extra_floats), y_bad says whether any observation is
 // non-finite (initial_guess.hpp:131-139).
 template <int MODEL>
 __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, GridInfo grid,
@@ -427,26 +440,36 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
         const float a3 = 3.0f * grid.guess_spacing, a2 = 2.0f * ((rr < lr) ? rr : lr);
         const float separation = (a3 < a2) ? a2 : a3;
         // The prominent local maximum of largest smoothed value, earliest on
-        // ties (initial_guess.hpp:203-217): every lane tests its own
-        // candidates (only those that would beat its best so far), then one
-        // wave reduction picks (largest value, smallest index).
+        // ties (initial_guess.hpp:203-217): candidates are taken in
+        // increasing index, so a later one must be strictly larger to win.
         const float t = 3.0f * noise;
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
-        for (int j = 1 + lane; j + 1 < m; j += 64) {
-            const float v = smooth_at(s, m, j);
-            if (!(v > v_best) || v <= smooth_at(s, m, j - 1) || v < smooth_at(s, m, j + 1))
-                continue;
-            const int idx = raw_peak_dev(s, j);
-            const int bins = idx > first ? idx - first : first - idx;
-            if (bins < 3 || fabsf(x[idx] - primary.center) < separation)
-                continue;
-            if (!prominent_dev(s, m, j, v, t))
-                continue;
-            v_best = v;
-            j_best = j;
+        for (int base = 1; base + 1 < m; base += 64) {
+            const int j = base + lane;
+            float v = 0.0f;
+            bool cand = false;
+            if (j + 1 < m) {
+                v = smooth_at(s, m, j);
+                cand = v > v_best && v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1);
+                if (cand) {
+                    const int idx = raw_peak_dev(s, j);
+                    const int bins = idx > first ? idx - first : first - idx;
+                    cand = !(bins < 3 || fabsf(x[idx] - primary.center) < separation);
+                }
+            }
+            // This step's candidates in increasing index, each tested by the
+            // whole wave; one that is not above the best so far cannot win.
+            for (uint64_t mask = __ballot(cand); mask; mask &= mask - 1) {
+                const int l = __builtin_ctzll(mask);
+                const float h = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+                if (h > v_best && prominent_wave(s, m, base + l, h, t, lane)) {
+                    v_best = h;
+                    j_best = base + l;
+                }
+            }
         }
-        wave_argmax(v_best, j_best);
         int second = j_best == 0x7fffffff ? -1 : raw_peak_dev(s, j_best);
         if (second < 0)
             return finish();

@@ -97,8 +97,8 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
             d.points != grid_table[d.grid].points)
             return "fit " + std::to_string(i) +
                    ": model, grid or config out of range, or M differs from its grid";
-        if (d.p0_row == 0xffffffffu && d.points > 13000)
-            return "the device initial guess supports M up to 13,000 points; pass p0";
+        if (d.p0_row == 0xffffffffu && d.points > kGuessMaxPoints)
+            return "the device initial guess supports M up to 12,288 points; pass p0";
         out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;
         out.any_p0 = out.any_p0 || d.p0_row != 0xffffffffu;
         out.out_rows = std::max(out.out_rows, d.out_row + 1);
@@ -163,8 +163,7 @@ hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice 
     const RaggedRun &r = layout.runs[i];
     hipError_t e;
     if (r.guesses) {
-        const uint32_t count = r.max_points / 10 > 1 ? r.max_points / 10 : 1;
-        const uint32_t sort_n = guess_sort_floats(count);
+        const uint32_t sort_n = guess_extra_floats(int(r.model), r.max_points);
         if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset, r.count, dev.grids,
                               dev.info, io.x, io.y, dev.guess, r.max_points, sort_n, s)) !=
             hipSuccess)

@@ -225,12 +225,11 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
 
     const float *init = p0;
     if (!init) {
-        const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-        const uint32_t sort_n = guess_sort_floats(count);
+        const uint32_t sort_n = guess_extra_floats(int(model), m);
         const size_t lds = size_t(m + sort_n) * sizeof(float);
         if (lds > 65536)
             return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "the device initial guess supports M up to 13,000 points; pass p0");
+                        "the device initial guess supports M up to 12,288 points; pass p0");
         float *guess = nullptr;
         BF_TRY(scratch.get(reinterpret_cast<void **>(&guess), size_t(n) * P * sizeof(float)),
                "hipMallocAsync(guess)");
@@ -433,12 +432,11 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     std::lock_guard<std::mutex> lock(rt.host_calls);
     hipStream_t s = rt.stream;
     const size_t P = size_t(kParams[model]);
-    const uint32_t count = m / 10 > 1 ? m / 10 : 1;
-    const uint32_t sort_n = guess_sort_floats(count);
+    const uint32_t sort_n = guess_extra_floats(int(model), m);
     const size_t lds = size_t(m + sort_n) * sizeof(float);
     if (lds > 65536)
         return fail(err, errlen, BF_INVALID_ARGUMENT,
-                    "the device initial guess supports M up to 13,000 points");
+                    "the device initial guess supports M up to 12,288 points");
     Scratch scratch(s);
     float *dx, *dy, *dp, *db;
     int32_t *dg;

@@ -132,6 +132,39 @@ def test_guess_bitwise_long_windows(hip, oracle_port, model, m):
         assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
 
 
+@pytest.mark.parametrize("m", [700, 2048, 4096])
+def test_two_peak_guess_prominence_walks(hip, oracle_port, m):
+    """The two-peak guess's search for a second prominent maximum on data
+    that make it walk far: one peak only, pure noise, and second peaks from
+    well below to well above three noise scales, on absolute noise. The
+    walks skip whole 64-point blocks; the result must equal the oracle's
+    point-by-point walk bit for bit."""
+    rng = np.random.default_rng(m)
+    n = 96
+    x = np.linspace(-10, 10, m).astype(np.float32)
+    a1 = rng.uniform(0.5, 5, n)
+    c1 = rng.uniform(-8, 8, n)
+    w1 = rng.uniform(0.05, 1.5, n)
+    kind = np.arange(n) % 4
+    a2 = np.where(kind == 0, 0.0, np.where(kind == 1, 0.0, rng.uniform(0.005, 0.2, n)))
+    a1 = np.where(kind == 1, 0.0, a1)  # pure noise rows
+    c2 = rng.uniform(-9, 9, n)
+    w2 = rng.uniform(0.05, 1.0, n)
+
+    def lor(a, c, w):
+        return a[:, None] / (1 + ((x[None, :] - c[:, None]) / w[:, None]) ** 2)
+
+    y = lor(a1, c1, w1) + lor(a2, c2, w2) + rng.normal(0, 0.01, (n, m))
+    y = y.astype(np.float32)
+    got = device_guess(x, y, "two_lorentzian")
+    ref = oracle_port.guess(x, y, "two_lorentzian", threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8)), m
+    # Both outcomes of the search occur in the set.
+    assert (ref[2] == 0).any() and (ref[2] == 3).any()
+
+
 def test_guess_bitwise_metric_and_doublets(hip, oracle_port):
     x, y, _, _ = datasets.config2(n=1 << 20, rows=4096)
     got = device_guess(x, y, "pseudo_voigt")
