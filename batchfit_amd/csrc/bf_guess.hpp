@@ -396,10 +396,22 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
         return finish();
     const float noise = 1.4826f * spread;
     int bad = 0;
-    for (int j = lane; j < m; j += 64) {
-        const float v = s[j] - (level + slope * (x[j] - left_x));
-        bad |= !finite_f(v);
-        s[j] = v;
+    if (paired || __float_as_uint(level) == 0x80000000u) {
+#pragma unroll 8
+        for (int j = lane; j < m; j += 64) {
+            const float v = s[j] - (level + slope * (x[j] - left_x));
+            bad |= !finite_f(v);
+            s[j] = v;
+        }
+    } else {
+        // slope is +0 and x finite, so level + slope·(x − left_x) is level
+        // itself unless level is −0: the grid is not read.
+#pragma unroll 8
+        for (int j = lane; j < m; j += 64) {
+            const float v = s[j] - level;
+            bad |= !finite_f(v);
+            s[j] = v;
+        }
     }
     bad = wave_or_i(bad);
     wave_sync();
@@ -408,6 +420,7 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
     // Strongest smoothed local maximum (initial_guess.hpp:170-182).
     float best = 0.0f;
     int first = 0x7fffffff;
+#pragma unroll 4
     for (int j = 1 + lane; j + 1 < m; j += 64) {
         const float v = smooth_at(s, m, j);
         if (v > best && v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1)) {
@@ -523,7 +536,22 @@ template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit)
     const int m = int(a.m);
     const float *y = a.y + fit * a.ys_fit;
     int bad = 0;
-    for (int j = lane; j < m; j += 64) {
+    // 16 loads per lane in flight at a time, then the remainder one by one.
+    constexpr int U = 16;
+    int j = lane;
+    for (; j + 64 * (U - 1) < m; j += 64 * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            v[u] = y[(j + 64 * u) * a.ys_pt];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            bad |= !finite_f(v[u]);
+            lds[j + 64 * u] = v[u];
+        }
+    }
+#pragma unroll 4
+    for (; j < m; j += 64) {
         const float v = y[j * a.ys_pt];
         bad |= !finite_f(v);
         lds[j] = v;
