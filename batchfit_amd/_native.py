@@ -58,7 +58,7 @@ class Launch(ctypes.Structure):
         ("mapping", ctypes.c_uint32),
         ("group_lanes", ctypes.c_uint32),
         ("point_major", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("streamed", ctypes.c_uint32),
         ("stream", ctypes.c_void_p),
     ]
 

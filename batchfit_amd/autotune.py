@@ -6,9 +6,10 @@ For every model and a ladder of point counts M, fits of that shape are
 generated in HBM (Philox generator, parameters as in the benchmark
 workloads), started from their true parameters so the guess plays no part,
 and solved by the LM kernel with each candidate mapping — 16, 32 or 64 lanes
-per fit, the register tile the smallest power of two ≥ M/G (streaming beyond
-32 points per lane) — in the production numerics (compensated partials, FP64
-solve). The kernel time is read from the library's events; the fastest
+per fit, with the points either in a register tile (the smallest power of two
+≥ M/G, up to 32 per lane) or read from memory on every pass — in the
+production numerics (compensated partials, FP64 solve). A fit's result
+depends on G and M only, so the table changes speed, not results. The kernel time is read from the library's events; the fastest
 mapping wins each (model, M). Adjacent winners are merged into ranges and,
 with --write, emitted as batchfit_amd/csrc/bf_tuned.hpp (the table both the
 uniform and the ragged paths use), followed by a rebuild. The measured table
@@ -33,7 +34,9 @@ from . import _native
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODELS = ("gaussian", "lorentzian", "pseudo_voigt", "two_lorentzian")
 LADDER = (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096)
-CANDIDATES = (16, 32, 64)
+# (lanes per fit, streamed): points in a register tile, or read from memory
+# on every pass.
+CANDIDATES = ((16, 0), (32, 0), (64, 0), (16, 1), (32, 1), (64, 1))
 RANGES = {  # parameters drawn per fit, and noise (σ relative to amplitude unless absolute)
     "gaussian": ([0.5, -3, 0.3], [5, 3, 1.5]),
     "lorentzian": ([0.5, -3, 0.3], [5, 3, 1.5]),
@@ -49,6 +52,18 @@ def tile(g: int, m: int) -> int:
     while t < r:
         t *= 2
     return t if t <= 32 else 0
+
+
+TIE = 0.03  # relative time difference treated as a tie
+
+
+def label(g: int, streamed: int) -> str:
+    return f"{g}s" if streamed else str(g)
+
+
+def unlabel(key) -> tuple:
+    key = str(key)
+    return (int(key[:-1]), 1) if key.endswith("s") else (int(key), 0)
 
 
 def measure(fits: int, repeat: int) -> dict:
@@ -77,12 +92,12 @@ def measure(fits: int, repeat: int) -> dict:
             cfg = _native.default_config()
             cfg.fits, cfg.points, cfg.numerics = fits, m, 5
             times = {}
-            for g in CANDIDATES:
-                r = tile(g, m)
-                if r == 0 and g != 64:
-                    continue  # only whole waves stream
+            for g, streamed in CANDIDATES:
+                if not streamed and tile(g, m) == 0:
+                    continue  # beyond the largest register tile
                 launch = _native.Launch()
                 launch.mapping, launch.group_lanes = _native.MAP_GROUP, g
+                launch.streamed = streamed
                 best = float("inf")
                 for _ in range(repeat + 1):
                     _native.check(lib.bf_hip_fit_device(
@@ -92,11 +107,11 @@ def measure(fits: int, repeat: int) -> dict:
                     t = (ctypes.c_double * 4)()
                     lib.bf_hip_last_timing(t, 4)
                     best = min(best, t[3])
-                times[g] = best
+                times[label(g, streamed)] = best
             winner = min(times, key=times.get)
             table[(model, m)] = (winner, times)
             print(f"{model:15s} M={m:5d} " +
-                  " ".join(f"G{g}={times[g]:.3f}ms" for g in sorted(times)) + f" -> {winner}",
+                  " ".join(f"G{k}={times[k]:.3f}ms" for k in times) + f" -> {winner}",
                   file=sys.stderr, flush=True)
             for a in [dx, dy, dt, dp0] + outs:
                 a.free()
@@ -104,27 +119,33 @@ def measure(fits: int, repeat: int) -> dict:
 
 
 def ranges(table: dict) -> dict:
-    """Per model: list of (largest M, lanes) covering every M. Adjacent ladder
-    points with the same winner are merged; between two winners the boundary
-    is the midpoint of their M, but never beyond what the lower winner's
-    largest register tile holds (32 points per lane)."""
+    """Per model: list of (largest M, lanes, streamed) covering every M.
+    Adjacent ladder points with the same winner are merged; between two
+    winners the boundary is the midpoint of their M (a mapping within 3 % of
+    the winner keeps its range), but a register-tile
+    winner with fewer than 64 lanes never extends beyond what its largest
+    tile holds (32 points per lane). The last range is open."""
     out = {}
     for model in MODELS:
-        pts = [(m, table[(model, m)][0]) for m in LADDER]
+        pts, prev = [], None
+        for m in LADDER:
+            winner, times = table[(model, m)]
+            # Keep the previous mapping while it is within TIE of the winner,
+            # so that measurement noise does not split the table.
+            if prev is not None and prev in times and times[prev] <= (1 + TIE) * times[winner]:
+                winner = prev
+            pts.append((m, unlabel(winner)))
+            prev = winner
         merged = []
-        for i, (m, g) in enumerate(pts):
-            if i + 1 < len(pts):
-                upper = (m + pts[i + 1][0]) // 2
-                if g < 64:
-                    upper = min(upper, 32 * g)
+        for i, (m, (g, st)) in enumerate(pts):
+            cap = 0xFFFFFFFF if (st or g == 64) else 32 * g
+            upper = min((m + pts[i + 1][0]) // 2, cap) if i + 1 < len(pts) else cap
+            if merged and merged[-1][1:] == (g, st):
+                merged[-1] = (upper, g, st)
             else:
-                upper = 0xFFFFFFFF if g == 64 else 32 * g
-            if merged and merged[-1][1] == g:
-                merged[-1] = (upper, g)
-            else:
-                merged.append((upper, g))
+                merged.append((upper, g, st))
         if merged[-1][0] != 0xFFFFFFFF:
-            merged.append((0xFFFFFFFF, 64))
+            merged.append((0xFFFFFFFF, 64, 0))
         out[model] = merged
     return out
 
@@ -135,7 +156,8 @@ def write_header(rng: dict, path: str) -> None:
         "// bf_tuned.hpp — generated by `python -m batchfit_amd.autotune --write`: lanes per",
         "// fit by (model, M), measured on gfx950 (LM kernel time of the production numerics",
         "// from true parameters; see batchfit_amd/autotune.py). Rows: gaussian, lorentzian,",
-        "// pseudo_voigt, two_lorentzian; each entry {largest M, lanes}, the last one open.",
+        "// pseudo_voigt, two_lorentzian; each entry {largest M, lanes, streamed}, the last",
+        "// one open.",
         "#pragma once",
         "",
         "#include <stdint.h>",
@@ -145,6 +167,7 @@ def write_header(rng: dict, path: str) -> None:
         "struct TunedRange {",
         "    uint32_t max_points;",
         "    int lanes;",
+        "    int streamed;  // 1: points read from memory on every pass",
         "};",
         f"constexpr int kTunedRanges = {width};",
         "constexpr TunedRange kTuned[4][kTunedRanges] = {",
@@ -153,7 +176,8 @@ def write_header(rng: dict, path: str) -> None:
         entries = list(rng[model])
         while len(entries) < width:
             entries.append(entries[-1])
-        body = ", ".join(f"{{{m if m != 0xFFFFFFFF else '0xffffffffu'}, {g}}}" for m, g in entries)
+        body = ", ".join(f"{{{m if m != 0xFFFFFFFF else '0xffffffffu'}, {g}, {st}}}"
+                         for m, g, st in entries)
         lines.append(f"    {{{body}}},  // {model}")
     lines += ["};", "", "}  // namespace bfhip", ""]
     with open(path, "w") as fh:
@@ -174,7 +198,7 @@ def main(argv=None) -> int:
         table = {}
         for key, v in saved["measured"].items():
             model, m = key.split(":")
-            table[(model, int(m))] = (v["winner"], {int(g): t for g, t in v["ms"].items()})
+            table[(model, int(m))] = (str(v["winner"]), dict(v["ms"]))
         args.fits = saved["fits"]
     else:
         table = measure(args.fits, args.repeat)

@@ -51,6 +51,11 @@ __device__ __forceinline__ float qnan_f() { return __uint_as_float(0x7fc00000u);
 // are hoisted into ModelConsts once per parameter vector.
 //   Gaussian     exp(-z²/2) = 2^(-z²·log2(e)/2)
 //   pseudo-Voigt exp(-4 ln2 z²) = 2^(-4 z²)   (exact rewrite of the exponent)
+// Contraction only within one source expression (a·b + c written together):
+// the code a model inlines into must not decide which products fuse, or the
+// same fit would round differently in the tile and the streamed passes.
+#pragma clang fp contract(on)
+
 template <int MODEL> struct ModelConsts;
 
 template <> struct ModelConsts<GAUSSIAN> {

@@ -107,15 +107,17 @@ template <int MODEL> hipError_t launch_guess_model(const GuessArgs &args, size_t
                                                    hipStream_t stream);
 template <int MODEL> hipError_t launch_eval_model(const EvalArgs &args, hipStream_t stream);
 
-// Lanes per fit by (model, M) from the measured table (bf_tuned.hpp). The
+// Lanes per fit and whether the points stream, by (model, M), from the
+// measured table (bf_tuned.hpp). The
 // same table drives uniform and ragged batches, so a fit's arithmetic does
 // not depend on how it was batched.
-inline int tuned_group(int model, uint32_t m) {
+inline TunedRange tuned_mapping(int model, uint32_t m) {
     for (const TunedRange &r : kTuned[model & 3])
         if (m <= r.max_points)
-            return r.lanes;
-    return 64;
+            return r;
+    return {0xffffffffu, 64, 0};
 }
+inline int tuned_group(int model, uint32_t m) { return tuned_mapping(model, m).lanes; }
 
 // Instantiated plans: lane-per-fit with every numerics form; 64-lane groups
 // with tiles 0 (stream), 2 … 32 and every numerics form; 16- and 32-lane
@@ -127,7 +129,7 @@ constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
     if (G == 64)
         return R == 0 || tile;
     if (G == 16 || G == 32)
-        return R >= 2 && tile && comp == mixed;
+        return (R == 0 || tile) && comp == mixed;
     return false;
 }
 

@@ -27,6 +27,11 @@
 
 #include "bf_guess.hpp"
 
+// Contraction only within one source expression, so that a fit's rounding
+// does not depend on the code around the point loop (register tile or
+// streamed, uniform or ragged launch).
+#pragma clang fp contract(on)
+
 namespace bfhip {
 
 struct FitArgs {
@@ -277,6 +282,15 @@ template <int G, int R, int CHUNK = 16> struct Points {
             }
             if (k)
                 flush();
+            if constexpr (G > 1) {
+                // The register tile flushes every block that holds a point of
+                // the group, so a lane whose points end one block early
+                // flushes once more: a fit's arithmetic depends on (G, M)
+                // only, whether its points stream or sit in registers.
+                const int mine = (m - gl + G - 1) / G;
+                if ((mine + kBlock - 1) / kBlock < (m + kBlock * G - 1) / (kBlock * G))
+                    flush();
+            }
         }
     }
     template <typename F> __device__ __forceinline__ void each(F &&f) const {

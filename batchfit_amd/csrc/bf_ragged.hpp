@@ -198,30 +198,32 @@ template <int MODEL> __global__ void __launch_bounds__(64) guess_ragged_kernel(
     guess_fit<MODEL>(g, 0);
 }
 
-// Mapping classes of a ragged batch: lanes per fit G (from the measured
-// table, as for uniform batches) and a register tile R large enough for the
-// fit's M — 16 or 32 points per lane, or the streamed form with whole waves
-// beyond 2048 points. The arithmetic of a fit depends on (G, M) only, so the
-// class's R does not change its result.
+// Mapping classes of a ragged batch: lanes per fit G and whether the points
+// stream (R = 0) or sit in a register tile of R = 16 or 32 per lane, chosen
+// from the measured table as for uniform batches. The arithmetic of a fit
+// depends on (G, M) only, so the class's R does not change its result.
 struct RaggedClass {
     int G, R;
     uint32_t max_points;
 };
-constexpr int kRaggedClassCount = 7;
+constexpr int kRaggedClassCount = 9;
 constexpr RaggedClass kRaggedClasses[kRaggedClassCount] = {
-    {16, 16, 256}, {16, 32, 512}, {32, 16, 512}, {32, 32, 1024},
-    {64, 16, 1024}, {64, 32, 2048}, {64, 0, 0xffffffffu}};
+    {16, 16, 256},  {16, 32, 512},         {32, 16, 512},         {32, 32, 1024},
+    {64, 16, 1024}, {64, 32, 2048},        {64, 0, 0xffffffffu},  {16, 0, 0xffffffffu},
+    {32, 0, 0xffffffffu}};
 // Enough queue counters for every (model, class) run.
-constexpr int kRaggedCounters = 32;
+constexpr int kRaggedCounters = 48;
 static_assert(4 * kRaggedClassCount <= kRaggedCounters, "one counter per run");
 inline int ragged_class_of(uint32_t model, uint32_t m) {
-    int g = tuned_group(int(model), m);
-    if (g < 64 && m > 32u * uint32_t(g))
-        g = 64;  // beyond the largest tile of a partial wave: whole waves
-    for (int c = 0; c < kRaggedClassCount; ++c)
-        if (kRaggedClasses[c].G == g && m <= kRaggedClasses[c].max_points)
+    const TunedRange t = tuned_mapping(int(model), m);
+    // Beyond 32 points per lane the points stream, as in a uniform launch.
+    const bool streamed = t.streamed || m > 32u * uint32_t(t.lanes);
+    for (int c = 0; c < kRaggedClassCount; ++c) {
+        const RaggedClass &k = kRaggedClasses[c];
+        if (k.G == t.lanes && (streamed ? k.R == 0 : (k.R > 0 && m <= k.max_points)))
             return c;
-    return kRaggedClassCount - 1;
+    }
+    return 6;  // 64 lanes, streamed (not reached for a valid table)
 }
 
 // One class of a ragged batch: a slice of the work order.

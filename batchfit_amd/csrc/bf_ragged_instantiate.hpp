@@ -29,6 +29,8 @@ hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hip
     case 4: return ragged_go<MODEL, 4, MIXED>(args, blocks, stream);
     case 5: return ragged_go<MODEL, 5, MIXED>(args, blocks, stream);
     case 6: return ragged_go<MODEL, 6, MIXED>(args, blocks, stream);
+    case 7: return ragged_go<MODEL, 7, MIXED>(args, blocks, stream);
+    case 8: return ragged_go<MODEL, 8, MIXED>(args, blocks, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -48,7 +50,9 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
         {&ragged_symbol<BF_MODEL, 3, false>, &ragged_symbol<BF_MODEL, 3, true>},
         {&ragged_symbol<BF_MODEL, 4, false>, &ragged_symbol<BF_MODEL, 4, true>},
         {&ragged_symbol<BF_MODEL, 5, false>, &ragged_symbol<BF_MODEL, 5, true>},
-        {&ragged_symbol<BF_MODEL, 6, false>, &ragged_symbol<BF_MODEL, 6, true>}};
+        {&ragged_symbol<BF_MODEL, 6, false>, &ragged_symbol<BF_MODEL, 6, true>},
+        {&ragged_symbol<BF_MODEL, 7, false>, &ragged_symbol<BF_MODEL, 7, true>},
+        {&ragged_symbol<BF_MODEL, 8, false>, &ragged_symbol<BF_MODEL, 8, true>}};
     if (cls < 0 || cls >= kRaggedClassCount)
         return hipErrorInvalidValue;
     const void *sym = table[cls][mixed ? 1 : 0]();

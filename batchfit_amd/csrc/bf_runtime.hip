@@ -85,9 +85,13 @@ int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *
         plan.R = 0;
         return BF_OK;
     }
-    int g = tuned_group(int(model), m);
+    const TunedRange tuned = tuned_mapping(int(model), m);
+    int g = tuned.lanes;
+    bool streamed = mapping == BF_MAP_AUTO && tuned.streamed;
     if (mapping == BF_MAP_GROUP && launch->group_lanes)
         g = int(launch->group_lanes);
+    if (mapping == BF_MAP_GROUP)
+        streamed = launch->streamed != 0;
     else if (mapping != BF_MAP_GROUP && mapping != BF_MAP_AUTO)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown mapping");
     if (mapping == BF_MAP_AUTO && g != 64 && !plan_instantiated(g, 2, plan.comp, plan.mixed))
@@ -96,12 +100,8 @@ int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *
         return fail(err, errlen, BF_INVALID_ARGUMENT, "group_lanes must be 16, 32 or 64");
     const uint32_t per_lane = (m + uint32_t(g) - 1) / uint32_t(g);
     int r = pow2_ceil(per_lane < 2 ? 2 : per_lane);
-    if (r > kMaxTile) {
-        if (g != 64)
-            return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "M exceeds the register tile of this group size; use 64 lanes");
+    if (r > kMaxTile || streamed)
         r = 0;
-    }
     plan.G = g;
     plan.R = r;
     if (!plan_instantiated(plan.G, plan.R, plan.comp, plan.mixed))

@@ -137,8 +137,8 @@ def test_two_peak_guess_prominence_walks(hip, oracle_port, m):
     """The two-peak guess's search for a second prominent maximum on data
     that make it walk far: one peak only, pure noise, and second peaks from
     well below to well above three noise scales, on absolute noise. The
-    walks skip whole 64-point blocks; the result must equal the oracle's
-    point-by-point walk bit for bit."""
+    wave tests each candidate 64 points of a side at a time; the result must
+    equal the oracle's point-by-point walk bit for bit."""
     rng = np.random.default_rng(m)
     n = 96
     x = np.linspace(-10, 10, m).astype(np.float32)
@@ -194,6 +194,40 @@ def test_fit_matches_fp32_oracle(hip, oracle_port, model, m, n):
             assert_fp32_gate(got, ref, f"{model} M={m} {mapping} numerics={numerics}")
 
 
+def fields_bitwise_equal(a, b, label):
+    for name, u, v in zip(a._fields, a, b):
+        assert np.array_equal(np.ascontiguousarray(u).view(np.uint8),
+                              np.ascontiguousarray(v).view(np.uint8)), f"{label}: {name}"
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("g,m", [(16, 37), (16, 100), (16, 257), (16, 512), (32, 300),
+                                 (32, 1000), (64, 129), (64, 1000), (64, 1500), (64, 2048)])
+def test_streamed_equals_register_tile(hip, model, g, m):
+    """Points read from memory on every pass give the register tile's result
+    bit for bit at the same (lanes, M), partial blocks and empty lanes
+    included, with and without uncertainties."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 97, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    for unc in (False, True):
+        a = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}", uncertainties=unc)
+        b = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}_streamed",
+                    uncertainties=unc)
+        fields_bitwise_equal(a, b, f"{model} G={g} M={m} uncertainties={unc}")
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("g,m", [(16, 1500), (32, 2600)])
+def test_streamed_groups_beyond_the_tile(hip, oracle_port, model, g, m):
+    """16- and 32-lane groups stream the points where no register tile holds
+    them (more than 32 points per lane)."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 65, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
+    got = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}")
+    assert_fp32_gate(got, ref, f"{model} M={m} group{g} (streamed)")
+
+
 @pytest.mark.parametrize("model", list(P))
 def test_fit_from_device_guess(hip, oracle_port, model):
     x, y, _, bounds, _ = datasets.comparison_set(model, 257, 129)
@@ -234,9 +268,9 @@ def test_metric_workload_vs_fp64_oracle(hip, oracle_port):
 @pytest.mark.parametrize("mapping", MAPPINGS)
 @pytest.mark.parametrize("numerics", NUMERICS)
 def test_failure_isolation(hip, golden, mapping, numerics):
-    """Includes a 16-lane request for M=1024, which has no register tile and
-    must be an argument error rather than a launch."""
-    """tests/metal_lm_checks.hpp:59-128 and cooperative_checks.hpp:203-257."""
+    """tests/metal_lm_checks.hpp:59-128 and cooperative_checks.hpp:203-257;
+    also a 16-lane request for M=1024, which no register tile holds, so the
+    points stream."""
     g = golden["failures"]
     x, y, p0 = datasets.isolation_fixture()
     kw = dict(mapping=mapping, numerics=numerics)
@@ -260,9 +294,14 @@ def test_failure_isolation(hip, golden, mapping, numerics):
     empty = hip.fit(np.linspace(0, 1, 32), np.empty((0, 32)), "gaussian", None, None, "hip", **kw)
     assert empty.params.shape == (0, 3) and empty.status.shape == (0,)
     if mapping == "group16":
+        # Beyond the largest register tile the 16 lanes read the points from
+        # memory: the same result as asking for that form.
         wide = np.linspace(-1, 1, 1024)
-        with pytest.raises(ValueError, match="register tile"):
-            hip.fit(wide, np.ones((2, 1024)), "gaussian", [1, 0, 1], None, "hip", **kw)
+        y_wide = np.ones((2, 1024))
+        a = hip.fit(wide, y_wide, "gaussian", [1, 0, 1], None, "hip", **kw)
+        kw_streamed = dict(kw, mapping="group16_streamed")
+        b = hip.fit(wide, y_wide, "gaussian", [1, 0, 1], None, "hip", **kw_streamed)
+        fields_bitwise_equal(a, b, "group16 M=1024")
 
 
 @pytest.mark.parametrize("model", list(P))

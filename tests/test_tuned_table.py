@@ -14,8 +14,8 @@ def parse_table():
     rows = re.findall(r"\{(\{.*\})\},\s*//\s*(\w+)", text)
     out = {}
     for body, model in rows:
-        pairs = re.findall(r"\{(0x[0-9a-f]+u|\d+),\s*(\d+)\}", body)
-        out[model] = [(int(m.rstrip("u"), 0), int(g)) for m, g in pairs]
+        entries = re.findall(r"\{(0x[0-9a-f]+u|\d+),\s*(\d+),\s*([01])\}", body)
+        out[model] = [(int(m.rstrip("u"), 0), int(g), int(st)) for m, g, st in entries]
     return out
 
 
@@ -23,25 +23,25 @@ def test_table_covers_every_model_and_m():
     table = parse_table()
     assert list(table) == list(autotune.MODELS)
     for model, entries in table.items():
-        assert entries[-1] == (0xFFFFFFFF, 64), model
-        bounds = [m for m, _ in entries]
+        assert entries[-1][0] == 0xFFFFFFFF, model
+        bounds = [m for m, _, _ in entries]
         assert bounds == sorted(bounds), model
-        for m, g in entries:
+        for m, g, streamed in entries:
             assert g in (16, 32, 64), model
-            if g < 64:
+            if g < 64 and not streamed:
                 assert m <= 32 * g, (model, m, g)
 
 
 def lookup(entries, m):
-    for bound, g in entries:
+    for bound, g, streamed in entries:
         if m <= bound:
-            return g
-    return 64
+            return autotune.label(g, streamed)
+    return "64"
 
 
 def test_ranges_merge_and_clamp():
     # Synthetic measurements: 16 lanes win to 512, 32 to 1024, then 64.
-    winners = {m: 16 if m <= 512 else 32 if m <= 1024 else 64 for m in autotune.LADDER}
+    winners = {m: "16" if m <= 512 else "32" if m <= 1024 else "64" for m in autotune.LADDER}
     table = {(model, m): (g, {g: 1.0}) for model in autotune.MODELS for m, g in winners.items()}
     rng = autotune.ranges(table)
     for model in autotune.MODELS:
@@ -49,18 +49,52 @@ def test_ranges_merge_and_clamp():
         for m in autotune.LADDER:
             assert lookup(entries, m) == winners[m], (model, m)
         # The 16-lane range ends where its largest tile does, not at the midpoint 640.
-        assert entries[0] == (512, 16)
-        assert entries[-1] == (0xFFFFFFFF, 64)
+        assert entries[0] == (512, 16, 0)
+        assert entries[-1] == (0xFFFFFFFF, 64, 0)
+
+
+def test_ranges_streamed_winners_are_open():
+    # Streamed 16-lane groups win from 1536 on: no tile cap, and the open
+    # range is theirs.
+    winners = {m: "16" if m <= 512 else "64" if m <= 1024 else "16s" for m in autotune.LADDER}
+    table = {(model, m): (g, {g: 1.0}) for model in autotune.MODELS for m, g in winners.items()}
+    rng = autotune.ranges(table)
+    for model in autotune.MODELS:
+        assert rng[model][-1] == (0xFFFFFFFF, 16, 1)
+        for m in autotune.LADDER:
+            assert lookup(rng[model], m) == winners[m], (model, m)
 
 
 def test_ranges_append_open_whole_wave_range():
-    table = {(model, m): (16 if m <= 512 else 32, {}) for model in autotune.MODELS
+    table = {(model, m): ("16" if m <= 512 else "32", {}) for model in autotune.MODELS
              for m in autotune.LADDER if m <= 1024}
     for model in autotune.MODELS:
         for m in autotune.LADDER:
-            table.setdefault((model, m), (32, {}))
-    # 32 lanes "win" beyond their tile capacity: the table must still end in 64.
+            table.setdefault((model, m), ("32", {}))
+    # 32-lane tiles "win" beyond their capacity: the table must still end open.
     rng = autotune.ranges(table)
     for model in autotune.MODELS:
-        assert rng[model][-1] == (0xFFFFFFFF, 64)
-        assert all(m <= 32 * g for m, g in rng[model] if g < 64)
+        assert rng[model][-1] == (0xFFFFFFFF, 64, 0)
+        assert all(m <= 32 * g for m, g, st in rng[model] if g < 64 and not st)
+
+
+def test_header_round_trip(tmp_path):
+    winners = {m: "16" if m <= 512 else "32s" for m in autotune.LADDER}
+    table = {(model, m): (g, {g: 1.0}) for model in autotune.MODELS for m, g in winners.items()}
+    rng = autotune.ranges(table)
+    path = tmp_path / "bf_tuned.hpp"
+    autotune.write_header(rng, str(path))
+    text = path.read_text()
+    assert "{512, 16, 0}, {0xffffffffu, 32, 1}" in text
+
+
+def test_ranges_ties_keep_the_previous_mapping():
+    # Alternating winners within 3 % of each other do not split the table.
+    table = {}
+    for model in autotune.MODELS:
+        for i, m in enumerate(autotune.LADDER):
+            a, b = (1.0, 1.02) if i % 2 else (1.02, 1.0)
+            table[(model, m)] = ("16" if a < b else "16s", {"16": a, "16s": b})
+    rng = autotune.ranges(table)
+    for model in autotune.MODELS:
+        assert rng[model] == [(0xFFFFFFFF, 16, 1)], rng[model]
