@@ -734,9 +734,14 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
 }
 
 // Occupancy floor per instantiation: three waves per SIMD (≤ 168 VGPRs) where
-// the state fits with at most a few spilled registers; otherwise unconstrained.
+// the state fits with at most a few spilled registers, two for the streamed
+// passes of groups (≤ 256 VGPRs, no tile to hold); otherwise unconstrained.
 constexpr int lm_min_waves(int model, int g, int r) {
-    return (model != TWO_LORENTZIAN && g == 64 && r > 0 && r <= 16) ? 3 : 1;
+    if (model == TWO_LORENTZIAN)
+        return 1;
+    if (g == 64 && r > 0 && r <= 16)
+        return 3;
+    return (g > 1 && r == 0) ? 3 : 1;
 }
 
 template <int MODEL, int G, int R, bool COMP, bool MIXED>

@@ -227,6 +227,14 @@ inline int ragged_class_of(uint32_t model, uint32_t m) {
 }
 
 // One class of a ragged batch: a slice of the work order.
+// Fits of a run whose M lies in one octave (2^k, 2^(k+1)] (all M <= 256
+// together), for the guess: each launch sizes its LDS for its own longest
+// fit, so short fits are not held to the occupancy of the longest.
+struct GuessSlice {
+    uint32_t offset, count;  // within the run's slice of the order array
+    uint32_t max_points;
+};
+
 struct RaggedRun {
     uint32_t model;
     int cls;
@@ -234,6 +242,7 @@ struct RaggedRun {
     uint32_t guesses;        // fits in the slice that start from the device guess
     uint32_t max_points;
     double cost;             // relative work, for ordering and branch balance
+    std::vector<GuessSlice> guess_slices;  // octaves that hold a guessed fit
 };
 
 // Host view of a ragged batch: validated descriptors bucketed by (model,

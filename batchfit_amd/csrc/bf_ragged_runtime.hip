@@ -111,10 +111,25 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
             continue;
         sort_longest_first(v, fit_table);
         RaggedRun r{uint32_t(b / kRaggedClassCount), b % kRaggedClassCount,
-                    uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0};
+                    uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0, {}};
         for (uint32_t i : v) {
             r.guesses += fit_table[i].p0_row == 0xffffffffu;
             r.max_points = std::max(r.max_points, fit_table[i].points);
+        }
+        // v is longest first, so each octave of M is a contiguous slice.
+        for (size_t a = 0; a < v.size();) {
+            const uint32_t top = fit_table[v[a]].points;
+            uint32_t below = 0;  // the octave is (below, 2·below]
+            if (top > 256)
+                for (below = 256; 2 * below < top; below *= 2) {
+                }
+            size_t end = a;
+            uint32_t guessed = 0;
+            for (; end < v.size() && fit_table[v[end]].points > below; ++end)
+                guessed += fit_table[v[end]].p0_row == 0xffffffffu;
+            if (guessed)
+                r.guess_slices.push_back({uint32_t(a), uint32_t(end - a), top});
+            a = end;
         }
         // Relative work: points × parameters-weighted passes (a P=8 fit costs
         // about three times a P=3 one per point).
@@ -162,11 +177,11 @@ hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice 
                        const RaggedIO &io, hipStream_t s) {
     const RaggedRun &r = layout.runs[i];
     hipError_t e;
-    if (r.guesses) {
-        const uint32_t sort_n = guess_extra_floats(int(r.model), r.max_points);
-        if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset, r.count, dev.grids,
-                              dev.info, io.x, io.y, dev.guess, r.max_points, sort_n, s)) !=
-            hipSuccess)
+    for (const GuessSlice &g : r.guess_slices) {
+        const uint32_t sort_n = guess_extra_floats(int(r.model), g.max_points);
+        if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset + g.offset, g.count,
+                              dev.grids, dev.info, io.x, io.y, dev.guess, g.max_points, sort_n,
+                              s)) != hipSuccess)
             return e;
     }
     RaggedArgs a{};
