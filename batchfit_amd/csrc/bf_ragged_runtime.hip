@@ -6,6 +6,7 @@
 #include "bf_ragged.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <vector>
 
@@ -315,6 +316,18 @@ template <typename T> hipError_t upload(T **dst, const T *src, size_t n, hipStre
     return e == hipSuccess ? hipStreamSynchronize(s) : e;
 }
 
+// Concurrent branches of a plan's graph: BF_HIP_RAGGED_BRANCHES (1 to 4,
+// default 4). One branch runs the runs one after another, so that a profile
+// shows each kernel's own time.
+int ragged_branches() {
+    static const int n = [] {
+        const char *v = std::getenv("BF_HIP_RAGGED_BRANCHES");
+        const int k = v ? std::atoi(v) : bf_ragged_plan_s::kBranches;
+        return k < 1 ? 1 : k > bf_ragged_plan_s::kBranches ? bf_ragged_plan_s::kBranches : k;
+    }();
+    return n;
+}
+
 int enqueue(bf_ragged_plan p, const float *x, const float *y, const float *p0, float *params,
             int32_t *status, uint32_t *iterations, float *chi2, float *cov, float *se,
             int32_t *us, char *err, size_t errlen) {
@@ -324,7 +337,7 @@ int enqueue(bf_ragged_plan p, const float *x, const float *y, const float *p0, f
     // branches (an event recorded on a capturing stream is not replayed).
     const RaggedIO io{x, y, p0, params, status, iterations, chi2,
                       p->uncertainties ? cov : nullptr, se, us};
-    const RaggedFork fork{bf_ragged_plan_s::kBranches, p->side, p->fork, p->join};
+    const RaggedFork fork{ragged_branches(), p->side, p->fork, p->join};
     BF_TRY(ragged_enqueue(p->layout, dev, p->mixed, io, p->stream, &fork),
            "ragged batch enqueue");
     return BF_OK;
