@@ -3,14 +3,16 @@
     python -m batchfit_amd.autotune [--fits 65536] [--repeat 3] [--write]
 
 For every model and a ladder of point counts M, fits of that shape are
-generated in HBM (Philox generator, parameters as in the benchmark
-workloads), started from their true parameters so the guess plays no part,
-and solved by the LM kernel with each candidate mapping — 16, 32 or 64 lanes
+generated in HBM (Philox generator, parameters as in the ragged benchmark
+workload), started from the device guess as in production (so the spread
+of iteration counts between the fits that share a wave is measured too), and
+solved by the LM kernel with each candidate mapping — 16, 32 or 64 lanes
 per fit, with the points either in a register tile (the smallest power of two
 ≥ M/G, up to 32 per lane) or read from memory on every pass — in the
 production numerics (compensated partials, FP64 solve). A fit's result
-depends on G and M only, so the table changes speed, not results. The kernel time is read from the library's events; the fastest
-mapping wins each (model, M). Adjacent winners are merged into ranges and,
+depends on G and M only, so the table changes speed, not results. The LM
+kernel time is read from the library's events; the fastest mapping wins
+each (model, M). Adjacent winners are merged into ranges and,
 with --write, emitted as batchfit_amd/csrc/bf_tuned.hpp (the table both the
 uniform and the ragged paths use), followed by a rebuild. The measured table
 is printed as JSON.
@@ -37,12 +39,14 @@ LADDER = (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 
 # (lanes per fit, streamed): points in a register tile, or read from memory
 # on every pass.
 CANDIDATES = ((16, 0), (32, 0), (64, 0), (16, 1), (32, 1), (64, 1))
-RANGES = {  # parameters drawn per fit, and noise (σ relative to amplitude unless absolute)
-    "gaussian": ([0.5, -3, 0.3], [5, 3, 1.5]),
-    "lorentzian": ([0.5, -3, 0.3], [5, 3, 1.5]),
-    "pseudo_voigt": ([0.5, -3, 0.6, 0.0], [5, 3, 3.0, 1.0]),
-    "two_lorentzian": ([0.5, -5, 0.3, 0.5, 0, 0, 0, 0], [5, 0, 1.0, 5, 0, 0, 0, 0]),
+RANGES = {  # parameters drawn per fit (the ragged benchmark workload's ranges)
+    "gaussian": ([0.5, -3, 0.085], [5, 3, 0.85]),
+    "lorentzian": ([0.5, -3, 0.1], [5, 3, 1.0]),
+    "pseudo_voigt": ([0.5, -3, 0.2, 0.0], [5, 3, 2.0, 1.0]),
+    "two_lorentzian": ([0.5, -5, 0.2, 0.5, 0, 0, 0, 0], [5, 5, 1.5, 5, 0, 0, 0, 0]),
 }
+NOISE = {"gaussian": (0.01, True), "lorentzian": (0.01, True), "pseudo_voigt": (0.01, True),
+         "two_lorentzian": (0.01, False)}  # σ, relative to the amplitude or absolute
 P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
 
@@ -80,13 +84,11 @@ def measure(fits: int, repeat: int) -> dict:
         lo, hi = RANGES[model]
         for m in LADDER:
             x = np.linspace(-10, 10, m).astype(np.float32)
-            dx, dy, dt = DeviceArray.of(x), DeviceArray(fits * m * 4), DeviceArray(fits * 32)
-            synthesize(model, fits, dx, m, lo, hi, dy, noise=0.01, noise_relative=True, seed=41,
-                       layout="doublet" if model == "two_lorentzian" else "independent",
-                       separation=(3.0, 6.0),
-                       truth=dt)
-            truth = dt.download(np.empty((fits, 8), np.float32))[:, :p].copy()
-            dp0 = DeviceArray.of(truth)
+            dx, dy = DeviceArray.of(x), DeviceArray(fits * m * 4)
+            noise, relative = NOISE[model]
+            synthesize(model, fits, dx, m, lo, hi, dy, noise=noise, noise_relative=relative,
+                       seed=41, layout="doublet" if model == "two_lorentzian" else "independent",
+                       separation=(0.5, 3.0))
             outs = [DeviceArray(fits * p * 4), DeviceArray(fits * 4), DeviceArray(fits * 4),
                     DeviceArray(fits * 4)]
             cfg = _native.default_config()
@@ -102,7 +104,7 @@ def measure(fits: int, repeat: int) -> dict:
                 for _ in range(repeat + 1):
                     _native.check(lib.bf_hip_fit_device(
                         _native.MODELS[model], ctypes.byref(cfg), ctypes.byref(launch), dx.ptr,
-                        dy.ptr, dp0.ptr, *[o.ptr for o in outs], None, None, None, err,
+                        dy.ptr, None, *[o.ptr for o in outs], None, None, None, err,
                         len(err)), err)
                     t = (ctypes.c_double * 4)()
                     lib.bf_hip_last_timing(t, 4)
@@ -113,7 +115,7 @@ def measure(fits: int, repeat: int) -> dict:
             print(f"{model:15s} M={m:5d} " +
                   " ".join(f"G{k}={times[k]:.3f}ms" for k in times) + f" -> {winner}",
                   file=sys.stderr, flush=True)
-            for a in [dx, dy, dt, dp0] + outs:
+            for a in [dx, dy] + outs:
                 a.free()
     return table
 

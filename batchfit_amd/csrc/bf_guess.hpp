@@ -43,11 +43,14 @@ __host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
 // edge medians fit in 64 KiB of LDS (12288 + 4096 floats).
 constexpr uint32_t kGuessMaxPoints = 12288;
 
-// LDS floats the guess of `model` needs beyond the signal of M points (the
-// sort buffer of its edge medians).
+// LDS floats the guess of `model` needs beyond the signal of M points: the
+// sort buffer of its edge medians and, for the two-peak model, one minimum
+// per 64-point step of the second-peak search (used after the medians, in
+// the same space).
 __host__ __device__ inline uint32_t guess_extra_floats(int model, uint32_t m) {
-    (void)model;
-    return guess_sort_floats(m / 10 > 1 ? m / 10 : 1);
+    const uint32_t sort = guess_sort_floats(m / 10 > 1 ? m / 10 : 1);
+    const uint32_t steps = model == TWO_LORENTZIAN ? (m + 63) / 64 : 0;
+    return sort > steps ? sort : steps;
 }
 
 enum : int32_t { GS_SUCCESS = 0, GS_INVALID = 1, GS_NO_PEAK = 2, GS_INSUFFICIENT = 3 };
@@ -99,6 +102,15 @@ __device__ __forceinline__ int wave_or_i(int v) {
     for (int o = 32; o > 0; o >>= 1)
         v |= __shfl_xor(v, o, 64);
     return v;
+}
+// Exact wave minimum of floats with no NaN (DPP and swizzle steps).
+__device__ __forceinline__ float wave_min_fx(float v) {
+    v = fminf(v, __int_as_float(xor_lane<1>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xor_lane<2>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xor_lane<4>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xor_lane<8>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xor_lane<16>(__float_as_int(v))));
+    return fminf(v, __int_as_float(xor_lane<32>(__float_as_int(v))));
 }
 __device__ __forceinline__ float wave_min_f(float v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -456,15 +468,37 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
         // ties (initial_guess.hpp:203-217): candidates are taken in
         // increasing index, so a later one must be strictly larger to win.
         const float t = 3.0f * noise;
+        // A side can pass only if it holds a point with h − v > t, so only if
+        // h − (minimum over any range that contains the side) > t. The
+        // minima of the 64-point steps from step c to the end (sfx[c]) and
+        // from the start through the current step (pmin) reject most
+        // candidates without a walk.
+        float *const sfx = buf;
+        {
+            float running = __uint_as_float(0x7f800000u);
+            for (int c = (m - 2) / 64; c >= 0; --c) {
+                const int j = 1 + 64 * c + lane;
+                running = fminf(running, wave_min_fx(j < m ? smooth_at(s, m, j)
+                                                           : __uint_as_float(0x7f800000u)));
+                if (lane == 0)
+                    sfx[c] = running;
+            }
+            wave_sync();
+        }
+        float pmin = s[0];
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
         for (int base = 1; base + 1 < m; base += 64) {
             const int j = base + lane;
             float v = 0.0f;
             bool cand = false;
-            if (j + 1 < m) {
+            if (j + 1 < m)
                 v = smooth_at(s, m, j);
-                cand = v > v_best && v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1);
+            pmin = fminf(pmin, wave_min_fx(j + 1 < m ? v : __uint_as_float(0x7f800000u)));
+            const float right_min = sfx[(base - 1) / 64];
+            if (j + 1 < m) {
+                cand = v > v_best && v - pmin > t && v - right_min > t &&
+                       v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1);
                 if (cand) {
                     const int idx = raw_peak_dev(s, j);
                     const int bins = idx > first ? idx - first : first - idx;
