@@ -58,7 +58,7 @@ def tile(g: int, m: int) -> int:
     return t if t <= 32 else 0
 
 
-TIE = 0.03  # relative time difference treated as a tie
+SWITCH = 0.05  # relative slowdown one change of mapping along the ladder must save
 
 
 def label(g: int, streamed: int) -> str:
@@ -120,24 +120,47 @@ def measure(fits: int, repeat: int) -> dict:
     return table
 
 
+def choose(rows: list) -> list:
+    """One mapping per ladder point: the sequence that minimises the summed
+    relative slowdown (time / fastest − 1) plus SWITCH for every change of
+    mapping between neighbours, so that measurement noise does not split the
+    table into many small ranges (dynamic programming over the ladder)."""
+    labels = sorted({k for _, times in rows for k in times})
+    inf = float("inf")
+
+    def cost(times, k):
+        return times[k] / min(times.values()) - 1.0 if k in times else inf
+
+    best = {k: cost(rows[0][1], k) for k in labels}
+    back = []
+    for _, times in rows[1:]:
+        prev = min(best, key=best.get)
+        step, links = {}, {}
+        for k in labels:
+            stay, move = best[k], best[prev] + SWITCH
+            links[k] = k if stay <= move else prev
+            step[k] = min(stay, move) + cost(times, k)
+        back.append(links)
+        best = step
+    k = min(best, key=best.get)
+    out = [k]
+    for links in reversed(back):
+        k = links[k]
+        out.append(k)
+    return out[::-1]
+
+
 def ranges(table: dict) -> dict:
     """Per model: list of (largest M, lanes, streamed) covering every M.
-    Adjacent ladder points with the same winner are merged; between two
-    winners the boundary is the midpoint of their M (a mapping within 3 % of
-    the winner keeps its range), but a register-tile
-    winner with fewer than 64 lanes never extends beyond what its largest
-    tile holds (32 points per lane). The last range is open."""
+    Mappings are chosen along the ladder by choose(); between two mappings
+    the boundary is the midpoint of their M, but a register-tile mapping with
+    fewer than 64 lanes never extends beyond what its largest tile holds (32
+    points per lane). The last range is open."""
     out = {}
     for model in MODELS:
-        pts, prev = [], None
-        for m in LADDER:
-            winner, times = table[(model, m)]
-            # Keep the previous mapping while it is within TIE of the winner,
-            # so that measurement noise does not split the table.
-            if prev is not None and prev in times and times[prev] <= (1 + TIE) * times[winner]:
-                winner = prev
-            pts.append((m, unlabel(winner)))
-            prev = winner
+        rows = [(m, table[(model, m)][1] or {table[(model, m)][0]: 1.0}) for m in LADDER]
+        picks = choose(rows)
+        pts = [(m, unlabel(k)) for (m, _), k in zip(rows, picks)]
         merged = []
         for i, (m, (g, st)) in enumerate(pts):
             cap = 0xFFFFFFFF if (st or g == 64) else 32 * g
