@@ -143,6 +143,27 @@ _MAPPINGS = {"auto": (0, 0, 0), "lane": (1, 0, 0), "group16": (2, 16, 0), "group
              "group32_streamed": (2, 32, 1), "group64_streamed": (2, 64, 1)}
 
 
+def plan(model, points, *, numerics=None, mapping="auto"):
+    """The mapping fit() uses for ``points`` per spectrum: ``(lanes, tile)``,
+    lanes per fit (1 = one lane per fit) and points per lane held in
+    registers (0 = read from memory on every pass). Host-only: no device is
+    needed and nothing is launched."""
+    if model not in _PARAMETERS:
+        raise ValueError(f"unknown model {model!r}; choose one of {tuple(_PARAMETERS)}")
+    if mapping not in _MAPPINGS:
+        raise ValueError(f"mapping must be one of {tuple(_MAPPINGS)}")
+    lib = _native.library()
+    launch = _native.Launch()
+    launch.mapping, launch.group_lanes, launch.streamed = _MAPPINGS[mapping]
+    lanes, tile = ctypes.c_uint32(), ctypes.c_uint32()
+    err = _native.error_buffer()
+    _native.check(lib.bf_hip_plan(_native.MODELS[model], int(points),
+                                  NUMERICS_DEFAULT if numerics is None else int(numerics),
+                                  ctypes.byref(launch), ctypes.byref(lanes), ctypes.byref(tile),
+                                  err, len(err)), err)
+    return lanes.value, tile.value
+
+
 def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncertainties=False,
         numerics=None, mapping="auto", max_iterations=100):
     """Fit shared x[M] and y[N,M]; return params[N,P], status/iterations/chi2[N].

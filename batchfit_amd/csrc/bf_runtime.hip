@@ -322,6 +322,20 @@ void bf_hip_default_config(bf_config *cfg) {
     }
 }
 
+int bf_hip_plan(uint32_t model, uint32_t points, uint32_t numerics, const bf_launch *launch,
+                uint32_t *group_lanes, uint32_t *tile_points, char *err, size_t errlen) {
+    if (model > 3)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown model");
+    Plan plan{};
+    if (const int rc = choose_plan(model, points, numerics, launch, plan, err, errlen))
+        return rc;
+    if (group_lanes)
+        *group_lanes = uint32_t(plan.G);
+    if (tile_points)
+        *tile_points = uint32_t(plan.R);
+    return BF_OK;
+}
+
 int bf_hip_set_timing(int enable) {
     g_timing.store(enable != 0);
     return BF_OK;

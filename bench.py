@@ -151,6 +151,9 @@ def bench_metric(S, args):
         return elapsed, np.array(kernel_ms), np.array(step_ms)
 
     prod = native.Launch()
+    lanes, tile = batchfit_amd.plan("pseudo_voigt", m)
+    form = (f"{lanes} lanes per fit, " +
+            (f"{tile}-point register tile" if tile else "points read from memory every pass"))
     cfg = S.config("pseudo_voigt", n, m, batchfit_amd.NUMERICS_DEFAULT, bounds)
     elapsed, kms, sms = run(cfg, prod, args.steps, args.warmup)
     status = dstatus.download(np.empty(n, np.int32))
@@ -173,7 +176,7 @@ def bench_metric(S, args):
             "bound": "mfma", "engine": "VALU FP32 (MFMA unused: P<=8 normal equations)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(),
-            "kernel": "lm_kernel<pseudo_voigt, 64 lanes, 16-point tile, compensated, FP64 solve>",
+            "kernel": f"lm_kernel<pseudo_voigt, {form}, compensated, FP64 solve>",
             "algorithmic_flop_per_launch": flops_lm,
             "algorithmic_bytes_per_launch": n * (4 * m + 16 + 12),
             "kernel_ms_hip_events": lm_ms,
@@ -185,6 +188,7 @@ def bench_metric(S, args):
         },
         "outcome": {"status_counts": np.bincount(status, minlength=6).tolist(),
                     "mean_iterations": float(iterations.mean())},
+        "mapping": form,
     }
     if "plain" in args.sections:
         plain = native.Launch()
@@ -660,7 +664,7 @@ def main():
                         "p0=None (device guess) seed 7",
             "fits": args.rows, "points": 1024,
             "numerics": "compensated FP32 partials + FP64 combine/solve/accept",
-            "mapping": "64 lanes per fit, y in registers",
+            "mapping": metric["mapping"],
         },
         "latency_ms": metric["latency_ms"],
         "kernel_ms": metric["kernel_ms"],

@@ -260,6 +260,14 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
 int bf_hip_model_eval(uint32_t model, uint32_t n, const float *x, const float *params,
                       float *value, float *jac, char *err, size_t errlen);
 
+/* The mapping a bf_hip_fit / bf_hip_fit_device call with these arguments
+ * uses, without launching anything (no device needed): lanes per fit (1 =
+ * one lane per fit) and the register tile in points per lane (0 = the points
+ * are read from memory on every pass). No reference counterpart: the
+ * reference's kernel choice is internal (include/batchfit/metal/batch.hpp:21-50). */
+int bf_hip_plan(uint32_t model, uint32_t points, uint32_t numerics, const bf_launch *launch,
+                uint32_t *group_lanes, uint32_t *tile_points, char *err, size_t errlen);
+
 /* Enable (1) or disable (0) kernel timing with HIP events; also enabled by the
  * environment variable BF_HIP_TIMING. */
 int bf_hip_set_timing(int enable);

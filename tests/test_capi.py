@@ -120,3 +120,27 @@ def test_server_and_plans_need_a_device():
                                err, len(err)) == _native.BF_INVALID_ARGUMENT
     assert lib.bf_serve_wait(None, 1, 0.0, None) == _native.BF_INVALID_ARGUMENT
     assert lib.bf_serve_destroy(None) == _native.BF_OK
+
+
+def test_plan_follows_the_tuned_table_without_a_device():
+    """bf_hip_plan is host logic: the mapping a launch would use."""
+    import re
+    from batchfit_amd import plan
+    header = open(os.path.join(ROOT, "batchfit_amd", "csrc", "bf_tuned.hpp")).read()
+    rows = re.findall(r"\{(\{.*\})\},\s*//\s*(\w+)", header)
+    table = {model: [(int(m.rstrip("u"), 0), int(g), int(st)) for m, g, st in
+                     re.findall(r"\{(0x[0-9a-f]+u|\d+),\s*(\d+),\s*([01])\}", body)]
+             for body, model in rows}
+    for model, entries in table.items():
+        for m in (5, 64, 100, 256, 300, 512, 1000, 1024, 2048, 4096, 9000):
+            bound, g, streamed = next(e for e in entries if m <= e[0])
+            per_lane = -(-m // g)
+            tile = max(2, 1 << (per_lane - 1).bit_length())
+            expect = (g, 0 if streamed or tile > 32 else tile)
+            assert plan(model, m) == expect, (model, m)
+    assert plan("pseudo_voigt", 1024, mapping="group64") == (64, 16)
+    assert plan("pseudo_voigt", 1024, mapping="group64_streamed") == (64, 0)
+    assert plan("gaussian", 1500, mapping="group16") == (16, 0)
+    assert plan("lorentzian", 100, mapping="lane") == (1, 0)
+    with pytest.raises(ValueError, match="numerics"):
+        plan("gaussian", 256, numerics=1, mapping="group16")
