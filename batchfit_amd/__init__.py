@@ -138,8 +138,9 @@ def _make_config(n, m, lower, upper, max_iterations, numerics):
     return cfg
 
 
-_MAPPINGS = {"auto": (0, 0, 0), "lane": (1, 0, 0), "group16": (2, 16, 0), "group32": (2, 32, 0),
-             "group64": (2, 64, 0), "wave": (2, 64, 0), "group16_streamed": (2, 16, 1),
+_MAPPINGS = {"auto": (0, 0, 0), "lane": (1, 0, 0), "group8": (2, 8, 0), "group16": (2, 16, 0),
+             "group32": (2, 32, 0), "group64": (2, 64, 0), "wave": (2, 64, 0),
+             "group8_streamed": (2, 8, 1), "group16_streamed": (2, 16, 1),
              "group32_streamed": (2, 32, 1), "group64_streamed": (2, 64, 1)}
 
 
@@ -175,7 +176,7 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
 
     Keyword-only extensions of the HIP device: ``numerics`` (bit0 compensated
     partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
-    ``mapping`` ("auto", "lane", "group16", "group32", "group64"; a "_streamed"
+    ``mapping`` ("auto", "lane", "group8", "group16", "group32", "group64"; a "_streamed"
     suffix reads the points from memory on every pass instead of registers,
     with the same results) and
     ``max_iterations`` (default 100, cpu/solver.hpp:27).

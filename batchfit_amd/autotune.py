@@ -6,7 +6,7 @@ For every model and a ladder of point counts M, fits of that shape are
 generated in HBM (Philox generator, parameters as in the ragged benchmark
 workload), started from the device guess as in production (so the spread
 of iteration counts between the fits that share a wave is measured too), and
-solved by the LM kernel with each candidate mapping — 16, 32 or 64 lanes
+solved by the LM kernel with each candidate mapping — 8, 16, 32 or 64 lanes
 per fit, with the points either in a register tile (the smallest power of two
 ≥ M/G, up to 32 per lane) or read from memory on every pass — in the
 production numerics (compensated partials, FP64 solve). A fit's result
@@ -38,7 +38,7 @@ MODELS = ("gaussian", "lorentzian", "pseudo_voigt", "two_lorentzian")
 LADDER = (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096)
 # (lanes per fit, streamed): points in a register tile, or read from memory
 # on every pass.
-CANDIDATES = ((16, 0), (32, 0), (64, 0), (16, 1), (32, 1), (64, 1))
+CANDIDATES = ((8, 0), (16, 0), (32, 0), (64, 0), (8, 1), (16, 1), (32, 1), (64, 1))
 RANGES = {  # parameters drawn per fit (the ragged benchmark workload's ranges)
     "gaussian": ([0.5, -3, 0.085], [5, 3, 0.85]),
     "lorentzian": ([0.5, -3, 0.1], [5, 3, 1.0]),
@@ -58,7 +58,7 @@ def tile(g: int, m: int) -> int:
     return t if t <= 32 else 0
 
 
-SWITCH = 0.05  # relative slowdown one change of mapping along the ladder must save
+SWITCH = 0.10  # relative slowdown one change of mapping along the ladder must save
 
 
 def label(g: int, streamed: int) -> str:

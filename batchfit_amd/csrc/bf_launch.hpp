@@ -128,7 +128,7 @@ constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
         return R == 0;
     if (G == 64)
         return R == 0 || tile;
-    if (G == 16 || G == 32)
+    if (G == 8 || G == 16 || G == 32)
         return (R == 0 || tile) && comp == mixed;
     return false;
 }

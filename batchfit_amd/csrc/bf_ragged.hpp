@@ -206,11 +206,11 @@ struct RaggedClass {
     int G, R;
     uint32_t max_points;
 };
-constexpr int kRaggedClassCount = 9;
+constexpr int kRaggedClassCount = 12;
 constexpr RaggedClass kRaggedClasses[kRaggedClassCount] = {
-    {16, 16, 256},  {16, 32, 512},         {32, 16, 512},         {32, 32, 1024},
-    {64, 16, 1024}, {64, 32, 2048},        {64, 0, 0xffffffffu},  {16, 0, 0xffffffffu},
-    {32, 0, 0xffffffffu}};
+    {16, 16, 256},  {16, 32, 512},        {32, 16, 512},        {32, 32, 1024},
+    {64, 16, 1024}, {64, 32, 2048},       {64, 0, 0xffffffffu}, {16, 0, 0xffffffffu},
+    {32, 0, 0xffffffffu}, {8, 16, 128},   {8, 32, 256},         {8, 0, 0xffffffffu}};
 // Enough queue counters for every (model, class) run.
 constexpr int kRaggedCounters = 48;
 static_assert(4 * kRaggedClassCount <= kRaggedCounters, "one counter per run");

@@ -31,6 +31,9 @@ hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hip
     case 6: return ragged_go<MODEL, 6, MIXED>(args, blocks, stream);
     case 7: return ragged_go<MODEL, 7, MIXED>(args, blocks, stream);
     case 8: return ragged_go<MODEL, 8, MIXED>(args, blocks, stream);
+    case 9: return ragged_go<MODEL, 9, MIXED>(args, blocks, stream);
+    case 10: return ragged_go<MODEL, 10, MIXED>(args, blocks, stream);
+    case 11: return ragged_go<MODEL, 11, MIXED>(args, blocks, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -52,7 +55,10 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
         {&ragged_symbol<BF_MODEL, 5, false>, &ragged_symbol<BF_MODEL, 5, true>},
         {&ragged_symbol<BF_MODEL, 6, false>, &ragged_symbol<BF_MODEL, 6, true>},
         {&ragged_symbol<BF_MODEL, 7, false>, &ragged_symbol<BF_MODEL, 7, true>},
-        {&ragged_symbol<BF_MODEL, 8, false>, &ragged_symbol<BF_MODEL, 8, true>}};
+        {&ragged_symbol<BF_MODEL, 8, false>, &ragged_symbol<BF_MODEL, 8, true>},
+        {&ragged_symbol<BF_MODEL, 9, false>, &ragged_symbol<BF_MODEL, 9, true>},
+        {&ragged_symbol<BF_MODEL, 10, false>, &ragged_symbol<BF_MODEL, 10, true>},
+        {&ragged_symbol<BF_MODEL, 11, false>, &ragged_symbol<BF_MODEL, 11, true>}};
     if (cls < 0 || cls >= kRaggedClassCount)
         return hipErrorInvalidValue;
     const void *sym = table[cls][mixed ? 1 : 0]();

@@ -14,12 +14,12 @@ struct TunedRange {
     int lanes;
     int streamed;  // 1: points read from memory on every pass
 };
-constexpr int kTunedRanges = 3;
+constexpr int kTunedRanges = 4;
 constexpr TunedRange kTuned[4][kTunedRanges] = {
-    {{320, 16, 0}, {0xffffffffu, 16, 1}, {0xffffffffu, 16, 1}},  // gaussian
-    {{224, 16, 0}, {0xffffffffu, 16, 1}, {0xffffffffu, 16, 1}},  // lorentzian
-    {{80, 16, 0}, {160, 32, 0}, {0xffffffffu, 16, 1}},  // pseudo_voigt
-    {{160, 16, 0}, {640, 64, 0}, {0xffffffffu, 64, 1}},  // two_lorentzian
+    {{40, 8, 0}, {112, 16, 0}, {896, 8, 1}, {0xffffffffu, 16, 1}},  // gaussian
+    {{80, 16, 0}, {160, 8, 0}, {1792, 8, 1}, {0xffffffffu, 32, 1}},  // lorentzian
+    {{40, 8, 0}, {160, 16, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // pseudo_voigt
+    {{80, 8, 0}, {224, 8, 1}, {640, 64, 0}, {0xffffffffu, 64, 1}},  // two_lorentzian
 };
 
 }  // namespace bfhip

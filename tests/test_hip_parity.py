@@ -18,7 +18,7 @@ import datasets
 
 pytestmark = pytest.mark.gpu
 
-MAPPINGS = ["lane", "group16", "group32", "group64"]
+MAPPINGS = ["lane", "group8", "group16", "group32", "group64"]
 NUMERICS = [0, 5]  # FP32 as the reference GPU kernels; compensated FP32 + FP64 solve
 P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
@@ -27,7 +27,7 @@ def mappings_for(m):
     """Mappings whose register tile (at most 32 points per lane) holds M."""
     return [mp for mp in MAPPINGS
             if mp in ("lane", "group64") or (mp == "group32" and m <= 1024)
-            or (mp == "group16" and m <= 512)]
+            or (mp == "group16" and m <= 512) or (mp == "group8" and m <= 256)]
 
 
 def assert_fp32_gate(got, ref, label):
@@ -201,7 +201,7 @@ def fields_bitwise_equal(a, b, label):
 
 
 @pytest.mark.parametrize("model", list(P))
-@pytest.mark.parametrize("g,m", [(16, 37), (16, 100), (16, 257), (16, 512), (32, 300),
+@pytest.mark.parametrize("g,m", [(8, 37), (8, 200), (16, 37), (16, 100), (16, 257), (16, 512), (32, 300),
                                  (32, 1000), (64, 129), (64, 1000), (64, 1500), (64, 2048)])
 def test_streamed_equals_register_tile(hip, model, g, m):
     """Points read from memory on every pass give the register tile's result
@@ -217,10 +217,10 @@ def test_streamed_equals_register_tile(hip, model, g, m):
 
 
 @pytest.mark.parametrize("model", list(P))
-@pytest.mark.parametrize("g,m", [(16, 1500), (32, 2600)])
+@pytest.mark.parametrize("g,m", [(8, 700), (16, 1500), (32, 2600)])
 def test_streamed_groups_beyond_the_tile(hip, oracle_port, model, g, m):
-    """16- and 32-lane groups stream the points where no register tile holds
-    them (more than 32 points per lane)."""
+    """8-, 16- and 32-lane groups stream the points where no register tile
+    holds them (more than 32 points per lane)."""
     x, y, p0, bounds, _ = datasets.comparison_set(model, 65, m)
     x, y = x.astype(np.float32), y.astype(np.float32)
     ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
@@ -313,7 +313,7 @@ def test_pathology_matrix(hip, golden, model):
     y = g[f"path_{model}_y"]
     for mapping in MAPPINGS:
         for numerics in (0, 1, 2, 3, 5, 7):
-            if mapping in ("group16", "group32") and numerics not in (0, 5):
+            if mapping in ("group8", "group16", "group32") and numerics not in (0, 5):
                 continue
             r = hip.fit(x, y, model, inits, None, "hip", mapping=mapping, numerics=numerics)
             assert list(r.status) == [4, 4, 3, 3, 3, 0, 5, 0, 3], (mapping, numerics, r.status)
