@@ -68,7 +68,7 @@ typedef struct bf_config {
 enum bf_mapping {
     BF_MAP_AUTO = 0,  /* tuned table by (model, M) */
     BF_MAP_LANE = 1,  /* one lane per fit, y streamed (point-major when transposed) */
-    BF_MAP_GROUP = 2  /* group_lanes lanes per fit, y held in registers */
+    BF_MAP_GROUP = 2  /* group_lanes lanes per fit, y in registers or streamed (see streamed) */
 };
 
 typedef struct bf_launch {
