@@ -357,6 +357,20 @@ template <int G> __device__ __forceinline__ int group_or(int v) {
         v |= xlane<32>(v);
     return v;
 }
+// Group maximum of floats (NaN ignored where one operand is a number, as
+// fmaxf); group_min_f below is its minimum counterpart.
+template <int G> __device__ __forceinline__ float group_max_f(float v) {
+    v = fmaxf(v, xlane_f<1>(v));
+    v = fmaxf(v, xlane_f<2>(v));
+    v = fmaxf(v, xlane_f<4>(v));
+    if constexpr (G >= 16)
+        v = fmaxf(v, xlane_f<8>(v));
+    if constexpr (G >= 32)
+        v = fmaxf(v, xlane_f<16>(v));
+    if constexpr (G >= 64)
+        v = fmaxf(v, xlane_f<32>(v));
+    return v;
+}
 template <int G> __device__ __forceinline__ int group_max_i(int v) {
     v = max(v, xlane<1>(v));
     v = max(v, xlane<2>(v));

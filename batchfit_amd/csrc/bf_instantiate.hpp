@@ -4,6 +4,8 @@
 
 #include "bf_launch.hpp"
 
+#include <algorithm>
+
 #if !defined(BF_MODEL) || !defined(BF_GROUP)
 #error "define BF_MODEL and BF_GROUP before including bf_instantiate.hpp"
 #endif
@@ -18,6 +20,16 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
     } else {
         constexpr int per_block = 256 / G;
         const uint32_t blocks = (args.n + per_block - 1) / per_block;
+        if constexpr (R == 0 && G > 1) {
+            if (args.pool && !args.cov) {
+                // Persistent waves that retire fits one by one (solve_pool).
+                const uint32_t waves = (args.n + kPoolRefill - 1) / kPoolRefill;
+                const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
+                hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED>), dim3(pool_blocks),
+                                   dim3(256), 0, stream, args);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((lm_kernel<MODEL, G, R, COMP, MIXED>), dim3(blocks), dim3(256), 0,
                            stream, args);
         return hipGetLastError();

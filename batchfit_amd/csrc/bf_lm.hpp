@@ -52,6 +52,7 @@ struct FitArgs {
     float initial_lambda, gradient_tolerance, step_tolerance, cost_tolerance;
     float lower[8], upper[8];
     uint32_t numerics;
+    uint32_t *pool;      // work counter of the retiring streamed kernel (zeroed), or null
     int32_t options_ok;  // λ and tolerances valid, bounds ordered/non-NaN/non-empty-side
     int32_t free_shape;  // cpu/solver.hpp:70-82
     float width_factor;  // cpu/solver.hpp:86-88
@@ -303,7 +304,7 @@ template <int G, int R, int CHUNK = 16> struct Points {
 // check of the linearisation (cpu/solver.hpp:182-190).
 template <int MODEL, int G, int R, bool COMP, typename S>
 __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL)> &pts, const float *p, S *terms,
-                                           bool &lin_ok) {
+                                           bool &lin_ok, float *yrange = nullptr) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI, T = Params<MODEL>::TERMS;
     ModelConsts<MODEL> mc;
     mc.set(p);
@@ -311,7 +312,13 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
 #pragma unroll
     for (int t = 0; t < T; ++t)
         acc[t].zero();
+    // With yrange: also the group's smallest and largest observation.
+    float ylo = __uint_as_float(0x7f800000u), yhi = -__uint_as_float(0x7f800000u);
     pts.each([&](float xj, float yj) {
+        if (yrange) {
+            ylo = fminf(ylo, yj);
+            yhi = fmaxf(yhi, yj);
+        }
         float jac[P];
         const float f = mc.eval(xj, jac);
         const float r = yj - f;
@@ -328,6 +335,14 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         for (int t = 0; t < T; ++t)
             acc[t].flush();
     });
+    if (yrange) {
+        if constexpr (G > 1) {
+            ylo = group_min_f<G>(ylo);
+            yhi = group_max_f<G>(yhi);
+        }
+        yrange[0] = ylo;
+        yrange[1] = yhi;
+    }
     lin_ok = true;
     if constexpr (sizeof(S) == 8) {
         // Each lane's (sum, carry) pair as one double, then one halving
@@ -733,6 +748,280 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
         covariance<MODEL, G, R, S>(a, pts, fit, gl, p, status, chi_out);
 }
 
+// Streamed groups that retire fits one by one. A wave owns a pool of fits
+// [next, end) (a slice of the batch, or tickets of a ragged run) and each of
+// its 64/G groups takes the next fit of the pool as soon as its own is done,
+// so no group idles while another fit of the wave still iterates. Every trip
+// of the loop makes one fused pass for every group that needs one (the others
+// pass over zero points), so the cross-lane reductions run in all lanes
+// together and the loop exit is wave-uniform. A fit goes through exactly the
+// steps of solve_fit (same passes, same order), so its bytes are the same.
+// The source of fits `src` provides: take(want) — called by every lane with
+// a request flag uniform within each group, returns a fit of the pool or
+// kNoFit; begin(fit, pts) — makes `fit` the group's current fit and sets its
+// points; args() / grid() — the current fit's arguments and grid facts; row()
+// — the current fit's output row in args().
+constexpr uint32_t kNoFit = 0xffffffffu;
+
+template <int MODEL, int G, bool COMP, typename S, typename Source>
+__device__ __forceinline__ void solve_pool(int gl, Source &src) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    constexpr float eps = 1.1920928955078125e-7f;
+    // A group that never gets a fit still takes part in every pass, over zero
+    // points: its lane index must be set, since the pass loops start there.
+    Points<G, 0, stream_chunk(MODEL)> pts;
+    pts.load(nullptr, nullptr, 0, gl, 0);
+    uint32_t fit = kNoFit;
+    bool iterating = false;  // false: the fit's first pass is next
+    float p[P], trial[P];
+    S A[TRI], g[P];
+    S chi = S(0), lambda = S(0), predicted = S(0);
+    bool lin_ok = false;
+    uint32_t it = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        p[k] = trial[k] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < TRI; ++t)
+        A[t] = S(0);
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        g[k] = S(0);
+
+    auto write = [&](const float *q, int32_t st, uint32_t iters, float c2) {
+        const FitArgs &a = src.args();
+        const size_t row = src.row();
+        if (gl == 0) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                a.params[row * P + k] = q[k];
+            a.status[row] = st;
+            a.iterations[row] = iters;
+            a.chi2[row] = c2;
+        }
+        fit = kNoFit;
+    };
+    auto finish = [&](int32_t st) { write(p, st, it, float(chi)); };
+    auto resolved = [&]() {
+        return resolved_status<MODEL>(p, src.grid().min_spacing, src.args().width_factor);
+    };
+
+    for (;;) {
+        const FitArgs &a = src.args();
+        // 1. The top of an iteration (cpu/solver.hpp:254-312) for the groups
+        //    that iterate: stops, damped solve, trial point, predicted gain.
+        bool pass = false;
+        if (fit != kNoFit && iterating) {
+            if (it >= a.max_iterations) {
+                finish(ST_MAX_ITERATIONS);
+            } else if (!lin_ok) {
+                finish(ST_INVALID);
+            } else {
+                bool pinned[P];
+                S gnorm = S(0);
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    pinned[k] = a.lower[k] == a.upper[k] || (p[k] == a.lower[k] && g[k] < S(0)) ||
+                                (p[k] == a.upper[k] && g[k] > S(0));
+                    if (!pinned[k])
+                        gnorm = smax(gnorm, sabs(g[k]));
+                }
+                if (gnorm <= S(a.gradient_tolerance)) {
+                    finish(resolved());
+                } else {
+                    S damped[TRI], step[P];
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        step[k] = pinned[k] ? S(0) : g[k];
+#pragma unroll
+                        for (int l = 0; l <= k; ++l)
+                            damped[tri(k, l)] = (pinned[k] || pinned[l]) ? (k == l ? S(1) : S(0))
+                                                                         : A[tri(k, l)];
+                        if (!pinned[k])
+                            damped[tri(k, k)] += lambda * A[tri(k, k)];
+                    }
+                    ++it;
+                    if (!chol_solve<S, P>(damped, step, (a.numerics & 2u) != 0)) {
+                        finish(ST_SINGULAR);
+                    } else {
+                        S step_norm = S(0), param_norm = S(0);
+                        bool finite_trial = true;
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            step_norm = smax(step_norm, sabs(step[k]));
+                            trial[k] = clampf(float(S(p[k]) + step[k]), a.lower[k], a.upper[k]);
+                            finite_trial = finite_trial && finite_f(trial[k]);
+                            step[k] = S(trial[k]) - S(p[k]);
+                            param_norm = smax(param_norm, S(fabsf(p[k])));
+                        }
+                        const S stol = S(a.step_tolerance);
+                        if (step_norm <= stol * (param_norm + stol) / smax(S(1), lambda)) {
+                            finish(resolved());
+                        } else {
+                            predicted = S(0);
+#pragma unroll
+                            for (int k = 0; k < P; ++k) {
+                                predicted += S(2) * g[k] * step[k];
+#pragma unroll
+                                for (int l = 0; l < P; ++l)
+                                    predicted -= step[k] * A[k >= l ? tri(k, l) : tri(l, k)] * step[l];
+                            }
+                            pass = finite_trial;
+                        }
+                    }
+                }
+            }
+        }
+        // 2. Groups without a fit take the next one and check its inputs
+        //    (cpu/solver.hpp:204-229); a valid one makes its first pass now.
+        const uint32_t next = src.take(fit == kNoFit);
+        if (fit == kNoFit && next != kNoFit) {
+            fit = next;
+            iterating = false;
+            src.begin(fit, pts);
+            const FitArgs &b = src.args();
+            bool bad = !b.options_ok || !src.grid().x_finite;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                p[k] = b.p0[src.row() * P + k];
+                bad = bad || !finite_f(p[k]);
+            }
+            if (bad) {
+                write(p, ST_INVALID, 0, qnan_f());
+            } else {
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    trial[k] = clampf(p[k], b.lower[k], b.upper[k]);
+                pass = true;
+            }
+        }
+        if (!__ballot(fit != kNoFit))
+            break;
+        // 3. One pass for every group, over zero points where none is due.
+        Points<G, 0, stream_chunk(MODEL)> due = pts;
+        due.m = pass ? pts.m : 0;
+        S terms[Params<MODEL>::TERMS];
+        bool pass_ok;
+        float yrange[2];
+        fused_pass<MODEL, G, 0, COMP, S>(due, trial, terms, pass_ok, yrange);
+        if (fit == kNoFit)
+            continue;
+        const FitArgs &c = src.args();
+        // 4. After the pass.
+        const S pass_chi = pass ? terms[TRI + P] : S(qnan_f());
+        if (!iterating) {
+            // First pass at the projected initial point (cpu/solver.hpp:230-245).
+            if (!finite_s(pass_chi)) {
+                // Invalid either way; a non-finite observation reports the
+                // initial parameters as given, as solve_fit does.
+                int flags = 0;
+                pts.each([&](float, float yj) { flags |= finite_f(yj) ? 0 : 1; });
+                flags = group_or<G>(flags);
+                if (flags) {
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                        p[k] = c.p0[src.row() * P + k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                        p[k] = trial[k];
+                }
+                write(p, ST_INVALID, 0, qnan_f());
+                continue;
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                p[k] = trial[k];
+            chi = pass_chi;
+            it = 0;
+            if (yrange[0] == yrange[1] && src.grid().varied_x && c.free_shape) {
+                finish(ST_DEGENERATE);
+                continue;
+            }
+#pragma unroll
+            for (int t = 0; t < TRI; ++t)
+                A[t] = terms[t];
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                g[k] = terms[TRI + k];
+            lin_ok = pass_ok;
+            lambda = S(c.initial_lambda);
+            iterating = true;
+            continue;
+        }
+        // Accept or reject the trial (cpu/solver.hpp:313-339).
+        const S lambda_cap = smax(S(c.initial_lambda), S(1.0f / eps));
+        const S reduction = chi - pass_chi;
+        const S threshold = S(c.cost_tolerance) * chi;
+        const bool small_change = finite_s(pass_chi) && finite_s(predicted) &&
+                                  predicted >= S(0) && sabs(reduction) <= threshold &&
+                                  predicted <= threshold / smax(S(1), lambda);
+        if (finite_s(pass_chi) && reduction > S(0) && finite_s(predicted) && predicted > S(0)) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                p[k] = trial[k];
+            chi = pass_chi;
+#pragma unroll
+            for (int t = 0; t < TRI; ++t)
+                A[t] = terms[t];
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                g[k] = terms[TRI + k];
+            lin_ok = pass_ok;
+            const S rho = reduction / predicted;
+            const S q = S(2) * rho - S(1);
+            const S factor = rho >= S(1) ? S(1) / S(3) : smax(S(1) / S(3), S(1) - q * q * q);
+            const S nl = lambda * factor;
+            lambda = nl < S(eps) ? S(eps) : (lambda_cap < nl ? lambda_cap : nl);
+        } else {
+            lambda = smin(lambda_cap, lambda * S(10));
+        }
+        if (small_change)
+            finish(resolved());
+    }
+}
+
+// The fits of a uniform batch as a source for solve_pool: a wave refills its
+// pool with kPoolRefill consecutive fits from the batch counter when it runs
+// dry (one atomic per refill, by lane 0), and serves its groups in lane order.
+constexpr uint32_t kPoolRefill = 32;
+
+template <int G> struct BatchPool {
+    const FitArgs &a;
+    GridInfo info;
+    uint32_t next = 0, end = 0, cur = kNoFit;
+    bool dry = false;
+    int lane, gl;
+    __device__ BatchPool(const FitArgs &args, int lane_, int gl_)
+        : a(args), info(*args.grid), lane(lane_), gl(gl_) {}
+    __device__ uint32_t take(bool want) {
+        const uint64_t leaders = __ballot(want && gl == 0);
+        if (!leaders)
+            return kNoFit;
+        if (next >= end && !dry) {
+            uint32_t base = 0;
+            if (lane == 0)
+                base = atomicAdd(a.pool, kPoolRefill);
+            base = __builtin_amdgcn_readfirstlane(base);
+            dry = base >= a.n;
+            next = dry ? a.n : base;
+            end = dry ? a.n : (a.n - base < kPoolRefill ? a.n : base + kPoolRefill);
+        }
+        const uint32_t rank = uint32_t(__popcll(leaders & ((1ull << (lane - gl)) - 1ull)));
+        const uint32_t avail = end - next, asked = uint32_t(__popcll(leaders));
+        const uint32_t got = next + rank;
+        next += asked < avail ? asked : avail;
+        return (want && rank < avail) ? got : kNoFit;
+    }
+    template <typename Pts> __device__ void begin(uint32_t fit, Pts &pts) {
+        cur = fit;
+        pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
+    }
+    __device__ const FitArgs &args() const { return a; }
+    __device__ const GridInfo &grid() const { return info; }
+    __device__ size_t row() const { return cur; }
+};
+
 // Occupancy floor per instantiation: three waves per SIMD (≤ 168 VGPRs) where
 // the state fits with at most a few spilled registers, two for the streamed
 // passes of groups (≤ 256 VGPRs, no tile to hold); otherwise unconstrained.
@@ -742,6 +1031,16 @@ constexpr int lm_min_waves(int model, int g, int r) {
     if (g == 64 && r > 0 && r <= 16)
         return 3;
     return (g > 1 && r == 0) ? 3 : 1;
+}
+
+// Streamed groups with retiring fits (no uncertainties): persistent waves over
+// the batch counter a.pool.
+template <int MODEL, int G, bool COMP, bool MIXED>
+__global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, 0)) lm_pool_kernel(FitArgs a) {
+    using S = typename std::conditional<MIXED, double, float>::type;
+    const int lane = int(threadIdx.x & 63);
+    BatchPool<G> src(a, lane, lane % G);
+    solve_pool<MODEL, G, COMP, S>(lane % G, src);
 }
 
 template <int MODEL, int G, int R, bool COMP, bool MIXED>

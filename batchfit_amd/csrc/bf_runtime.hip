@@ -28,8 +28,13 @@ namespace {
 // ---------------------------------------------------------------------------
 // Device kernels owned by the runtime.
 
-__global__ void __launch_bounds__(256) grid_kernel(const float *x, uint32_t m, GridInfo *out) {
+// Grid facts of x, and the zeroed work counter of the LM launch that follows
+// (pool may be null).
+__global__ void __launch_bounds__(256) grid_kernel(const float *x, uint32_t m, GridInfo *out,
+                                                   uint32_t *pool) {
     grid_block(x, m, out);
+    if (pool && threadIdx.x == 0)
+        *pool = 0;
 }
 
 // Fit-major [n,m] → point-major [m,n] (the layout of the reference GPU
@@ -214,10 +219,13 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         b = false;
     Scratch scratch(stream);
     GridInfo *grid = nullptr;
-    BF_TRY(scratch.get(reinterpret_cast<void **>(&grid), sizeof(GridInfo)), "hipMallocAsync(grid)");
+    // GridInfo, then the work counter of the retiring kernel at a 256-byte offset.
+    BF_TRY(scratch.get(reinterpret_cast<void **>(&grid), 512), "hipMallocAsync(grid)");
+    static_assert(sizeof(GridInfo) <= 256, "grid facts fit before the counter");
+    uint32_t *pool = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(grid) + 256);
     if (timing)
         BF_TRY(record(0, stream), "hipEventRecord");
-    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, stream, x, m, grid);
+    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, stream, x, m, grid, pool);
     BF_TRY(hipGetLastError(), "grid prologue launch");
     if (timing)
         BF_TRY(record(1, stream), "hipEventRecord");
@@ -272,6 +280,7 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     a.stderr_ = se;
     a.ustatus = us;
     a.grid = grid;
+    a.pool = pool;
     a.n = n;
     a.m = m;
     a.max_iterations = cfg->max_iterations;
@@ -463,7 +472,7 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     dg = reinterpret_cast<int32_t *>(db + n);
     BF_TRY(hipMemcpyAsync(dx, x, size_t(m) * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     BF_TRY(hipMemcpyAsync(dy, y, size_t(n) * m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, s, dx, m, grid);
+    hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, s, dx, m, grid, nullptr);
     BF_TRY(hipGetLastError(), "grid prologue launch");
     GuessArgs ga{dx, dy, (long long)m, 1, grid, n, m, m, sort_n, dp, db, dg};
     BF_TRY(launch_guess(model, ga, lds, s), "initial guess launch");
