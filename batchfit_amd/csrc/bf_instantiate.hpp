@@ -23,7 +23,9 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
         if constexpr (R == 0 && G > 1) {
             if (args.pool && !args.cov) {
                 // Persistent waves that retire fits one by one (solve_pool).
-                const uint32_t waves = (args.n + kPoolRefill - 1) / kPoolRefill;
+                // One refill per wave to start with, at most 4096 waves (more
+                // than the device holds at once).
+                const uint32_t waves = (args.n + pool_refill(G) - 1) / pool_refill(G);
                 const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
                 hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED>), dim3(pool_blocks),
                                    dim3(256), 0, stream, args);

@@ -807,6 +807,13 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
     };
 
     for (;;) {
+        // Every trip starts at this wave-wide test, a convergent operation the
+        // optimiser cannot duplicate into the paths that end the previous
+        // trip; so all lanes begin each trip together and the queue below is
+        // always read by the whole wave. The loop ends when no group holds a
+        // fit and the queue is empty.
+        if (!__ballot(fit != kNoFit) && src.dry)
+            break;
         const FitArgs &a = src.args();
         // 1. The top of an iteration (cpu/solver.hpp:254-312) for the groups
         //    that iterate: stops, damped solve, trial point, predicted gain.
@@ -895,8 +902,6 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
                 pass = true;
             }
         }
-        if (!__ballot(fit != kNoFit))
-            break;
         // 3. One pass for every group, over zero points where none is due.
         Points<G, 0, stream_chunk(MODEL)> due = pts;
         due.m = pass ? pts.m : 0;
@@ -904,127 +909,148 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
         bool pass_ok;
         float yrange[2];
         fused_pass<MODEL, G, 0, COMP, S>(due, trial, terms, pass_ok, yrange);
-        if (fit == kNoFit)
-            continue;
-        const FitArgs &c = src.args();
-        // 4. After the pass.
-        const S pass_chi = pass ? terms[TRI + P] : S(qnan_f());
-        if (!iterating) {
-            // First pass at the projected initial point (cpu/solver.hpp:230-245).
-            if (!finite_s(pass_chi)) {
-                // Invalid either way; a non-finite observation reports the
-                // initial parameters as given, as solve_fit does.
-                int flags = 0;
-                pts.each([&](float, float yj) { flags |= finite_f(yj) ? 0 : 1; });
-                flags = group_or<G>(flags);
-                if (flags) {
+        // 4. After the pass (no early exits: every path reaches the next trip's
+        //    test the same way).
+        if (fit != kNoFit) {
+            const FitArgs &c = src.args();
+            const S pass_chi = pass ? terms[TRI + P] : S(qnan_f());
+            if (!iterating) {
+                // First pass at the projected initial point (cpu/solver.hpp:230-245).
+                if (!finite_s(pass_chi)) {
+                    // Invalid either way; a non-finite observation reports the
+                    // initial parameters as given, as solve_fit does.
+                    int flags = 0;
+                    pts.each([&](float, float yj) { flags |= finite_f(yj) ? 0 : 1; });
+                    flags = group_or<G>(flags);
 #pragma unroll
                     for (int k = 0; k < P; ++k)
-                        p[k] = c.p0[src.row() * P + k];
+                        p[k] = flags ? c.p0[src.row() * P + k] : trial[k];
+                    write(p, ST_INVALID, 0, qnan_f());
                 } else {
 #pragma unroll
                     for (int k = 0; k < P; ++k)
                         p[k] = trial[k];
+                    chi = pass_chi;
+                    it = 0;
+                    if (yrange[0] == yrange[1] && src.grid().varied_x && c.free_shape) {
+                        finish(ST_DEGENERATE);
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < TRI; ++t)
+                            A[t] = terms[t];
+#pragma unroll
+                        for (int k = 0; k < P; ++k)
+                            g[k] = terms[TRI + k];
+                        lin_ok = pass_ok;
+                        lambda = S(c.initial_lambda);
+                        iterating = true;
+                    }
                 }
-                write(p, ST_INVALID, 0, qnan_f());
-                continue;
+            } else {
+                // Accept or reject the trial (cpu/solver.hpp:313-339).
+                const S lambda_cap = smax(S(c.initial_lambda), S(1.0f / eps));
+                const S reduction = chi - pass_chi;
+                const S threshold = S(c.cost_tolerance) * chi;
+                const bool small_change = finite_s(pass_chi) && finite_s(predicted) &&
+                                          predicted >= S(0) && sabs(reduction) <= threshold &&
+                                          predicted <= threshold / smax(S(1), lambda);
+                if (finite_s(pass_chi) && reduction > S(0) && finite_s(predicted) &&
+                    predicted > S(0)) {
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                        p[k] = trial[k];
+                    chi = pass_chi;
+#pragma unroll
+                    for (int t = 0; t < TRI; ++t)
+                        A[t] = terms[t];
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                        g[k] = terms[TRI + k];
+                    lin_ok = pass_ok;
+                    const S rho = reduction / predicted;
+                    const S q = S(2) * rho - S(1);
+                    const S factor =
+                        rho >= S(1) ? S(1) / S(3) : smax(S(1) / S(3), S(1) - q * q * q);
+                    const S nl = lambda * factor;
+                    lambda = nl < S(eps) ? S(eps) : (lambda_cap < nl ? lambda_cap : nl);
+                } else {
+                    lambda = smin(lambda_cap, lambda * S(10));
+                }
+                if (small_change)
+                    finish(resolved());
             }
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-                p[k] = trial[k];
-            chi = pass_chi;
-            it = 0;
-            if (yrange[0] == yrange[1] && src.grid().varied_x && c.free_shape) {
-                finish(ST_DEGENERATE);
-                continue;
-            }
-#pragma unroll
-            for (int t = 0; t < TRI; ++t)
-                A[t] = terms[t];
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-                g[k] = terms[TRI + k];
-            lin_ok = pass_ok;
-            lambda = S(c.initial_lambda);
-            iterating = true;
-            continue;
         }
-        // Accept or reject the trial (cpu/solver.hpp:313-339).
-        const S lambda_cap = smax(S(c.initial_lambda), S(1.0f / eps));
-        const S reduction = chi - pass_chi;
-        const S threshold = S(c.cost_tolerance) * chi;
-        const bool small_change = finite_s(pass_chi) && finite_s(predicted) &&
-                                  predicted >= S(0) && sabs(reduction) <= threshold &&
-                                  predicted <= threshold / smax(S(1), lambda);
-        if (finite_s(pass_chi) && reduction > S(0) && finite_s(predicted) && predicted > S(0)) {
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-                p[k] = trial[k];
-            chi = pass_chi;
-#pragma unroll
-            for (int t = 0; t < TRI; ++t)
-                A[t] = terms[t];
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-                g[k] = terms[TRI + k];
-            lin_ok = pass_ok;
-            const S rho = reduction / predicted;
-            const S q = S(2) * rho - S(1);
-            const S factor = rho >= S(1) ? S(1) / S(3) : smax(S(1) / S(3), S(1) - q * q * q);
-            const S nl = lambda * factor;
-            lambda = nl < S(eps) ? S(eps) : (lambda_cap < nl ? lambda_cap : nl);
-        } else {
-            lambda = smin(lambda_cap, lambda * S(10));
-        }
-        if (small_change)
-            finish(resolved());
     }
 }
 
-// The fits of a uniform batch as a source for solve_pool: a wave refills its
-// pool with kPoolRefill consecutive fits from the batch counter when it runs
-// dry (one atomic per refill, by lane 0), and serves its groups in lane order.
-constexpr uint32_t kPoolRefill = 32;
+// The work queue of a retiring kernel: a wave refills its pool with
+// pool_refill(G) consecutive task indices from a counter when it runs dry (one
+// atomic per refill, by lane 0) and serves its groups in lane order. Tasks
+// run from 0 to total.
+constexpr uint32_t pool_refill(int g) { return 2u * uint32_t(64 / g) < 8u ? 8u : 2u * uint32_t(64 / g); }
 
-template <int G> struct BatchPool {
-    const FitArgs &a;
-    GridInfo info;
-    uint32_t next = 0, end = 0, cur = kNoFit;
+template <int G> struct PoolQueue {
+    static constexpr uint32_t kRefill = pool_refill(G);
+    uint32_t *counter;
+    uint32_t total;
+    uint32_t next = 0, end = 0;
     bool dry = false;
     int lane, gl;
-    __device__ BatchPool(const FitArgs &args, int lane_, int gl_)
-        : a(args), info(*args.grid), lane(lane_), gl(gl_) {}
+    __device__ PoolQueue(uint32_t *counter_, uint32_t total_, int lane_, int gl_)
+        : counter(counter_), total(total_), lane(lane_), gl(gl_) {}
     __device__ uint32_t take(bool want) {
         const uint64_t leaders = __ballot(want && gl == 0);
         if (!leaders)
             return kNoFit;
-        if (next >= end && !dry) {
+        const uint32_t rank = uint32_t(__popcll(leaders & ((1ull << (lane - gl)) - 1ull)));
+        const uint32_t asked = uint32_t(__popcll(leaders));
+        // What is left of the pool goes to the first requests, a refill
+        // (kRefill ≥ 64/G, so one is enough) to the rest.
+        const uint32_t left_begin = next, left = end - next;
+        uint32_t fresh = 0;
+        if (asked > left && !dry) {
             uint32_t base = 0;
             if (lane == 0)
-                base = atomicAdd(a.pool, kPoolRefill);
+                base = atomicAdd(counter, kRefill);
             base = __builtin_amdgcn_readfirstlane(base);
-            dry = base >= a.n;
-            next = dry ? a.n : base;
-            end = dry ? a.n : (a.n - base < kPoolRefill ? a.n : base + kPoolRefill);
+            dry = base >= total;
+            next = dry ? total : base;
+            end = dry ? total : (total - base < kRefill ? total : base + kRefill);
+            fresh = end - next;
+        } else {
+            next += asked < left ? asked : left;
         }
-        const uint32_t rank = uint32_t(__popcll(leaders & ((1ull << (lane - gl)) - 1ull)));
-        const uint32_t avail = end - next, asked = uint32_t(__popcll(leaders));
-        const uint32_t got = next + rank;
-        next += asked < avail ? asked : avail;
-        return (want && rank < avail) ? got : kNoFit;
+        uint32_t got = kNoFit;
+        if (rank < left)
+            got = left_begin + rank;
+        else if (rank - left < fresh)
+            got = next + (rank - left);
+        if (fresh)
+            next += asked - left < fresh ? asked - left : fresh;
+        return want ? got : kNoFit;
     }
+};
+
+// The fits of a uniform batch as a source for solve_pool.
+template <int G> struct BatchPool : PoolQueue<G> {
+    const FitArgs &a;
+    GridInfo info;
+    uint32_t cur = kNoFit;
+    __device__ BatchPool(const FitArgs &args, int lane_, int gl_)
+        : PoolQueue<G>(args.pool, args.n, lane_, gl_), a(args), info(*args.grid) {}
     template <typename Pts> __device__ void begin(uint32_t fit, Pts &pts) {
         cur = fit;
-        pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
+        pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, this->gl, int(a.m));
     }
     __device__ const FitArgs &args() const { return a; }
     __device__ const GridInfo &grid() const { return info; }
     __device__ size_t row() const { return cur; }
 };
 
-// Occupancy floor per instantiation: three waves per SIMD (≤ 168 VGPRs) where
-// the state fits with at most a few spilled registers, two for the streamed
-// passes of groups (≤ 256 VGPRs, no tile to hold); otherwise unconstrained.
+// Occupancy floor per instantiation: three waves per SIMD (≤ 168 VGPRs) for
+// 64-lane tiles of up to 16 points and for the streamed passes of groups of
+// the single-peak models; otherwise unconstrained (the two-peak model's state
+// takes a wave per SIMD).
 constexpr int lm_min_waves(int model, int g, int r) {
     if (model == TWO_LORENTZIAN)
         return 1;

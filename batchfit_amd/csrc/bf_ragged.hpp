@@ -61,6 +61,7 @@ __device__ __forceinline__ FitArgs ragged_fit_args(const RaggedArgs &r, const bf
     if constexpr (G == 64)
         c = wave_uniform(c);
     FitArgs a;
+    a.pool = nullptr;
     a.x = r.x + r.grids[d.grid].x_offset;
     a.y = r.y + d.y_offset;
     a.ys_fit = 0;
@@ -168,6 +169,41 @@ __global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, R))
             }
         }
     }
+}
+
+// The fits of one streamed class of a ragged batch as a source for
+// solve_pool: tasks are the class's slice of the work order; each group holds
+// its current fit's arguments (scalar for whole-wave groups).
+template <int MODEL, int G> struct RaggedPool : PoolQueue<G> {
+    const RaggedArgs &r;
+    FitArgs fa;
+    GridInfo info;
+    __device__ RaggedPool(const RaggedArgs &args, int lane_, int gl_)
+        : PoolQueue<G>(args.cursor, args.count, lane_, gl_), r(args) {
+        fa.pool = nullptr;
+    }
+    template <typename Pts> __device__ void begin(uint32_t task, Pts &pts) {
+        bf_fit_desc d = r.fits[r.order[task]];
+        if constexpr (G == 64)
+            d = wave_uniform(d);
+        fa = ragged_fit_args<G>(r, d, MODEL, Params<MODEL>::P);
+        info = *fa.grid;
+        pts.load(fa.x, fa.y, 1, this->gl, int(fa.m));
+    }
+    __device__ const FitArgs &args() const { return fa; }
+    __device__ const GridInfo &grid() const { return info; }
+    __device__ size_t row() const { return 0; }
+};
+
+// A streamed class without uncertainties: persistent waves whose groups
+// retire fits one by one (solve_pool) from the class's counter.
+template <int MODEL, int G, bool MIXED>
+__global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, 0))
+    lm_ragged_pool_kernel(RaggedArgs r) {
+    using S = typename std::conditional<MIXED, double, float>::type;
+    const int lane = int(threadIdx.x & 63);
+    RaggedPool<MODEL, G> src(r, lane, lane % G);
+    solve_pool<MODEL, G, MIXED, S>(lane % G, src);
 }
 
 // Data-only guess of one class of a ragged batch, one wave per fit; LDS sized

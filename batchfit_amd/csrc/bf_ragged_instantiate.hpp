@@ -5,6 +5,8 @@
 
 #include "bf_ragged.hpp"
 
+#include <algorithm>
+
 #ifndef BF_MODEL
 #error "define BF_MODEL before including bf_ragged_instantiate.hpp"
 #endif
@@ -15,6 +17,17 @@ namespace {
 template <int MODEL, int CLS, bool MIXED>
 hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream) {
     constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    if constexpr (R == 0) {
+        if (!args.cov) {
+            // Streamed class, no uncertainties: retiring groups; one refill
+            // per wave to start with, at most 4096 waves.
+            const uint32_t waves = (args.count + pool_refill(G) - 1) / pool_refill(G);
+            const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
+            hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED>), dim3(pool_blocks),
+                               dim3(256), 0, stream, args);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>), dim3(blocks), dim3(256), 0,
                        stream, args);
     return hipGetLastError();
@@ -41,6 +54,14 @@ template <int MODEL, int CLS, bool MIXED> const void *ragged_symbol() {
     constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
     return reinterpret_cast<const void *>(&lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>);
 }
+// The retiring kernel of a streamed class (null for a tile class).
+template <int MODEL, int CLS, bool MIXED> const void *pool_symbol() {
+    constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    if constexpr (R == 0)
+        return reinterpret_cast<const void *>(&lm_ragged_pool_kernel<MODEL, G, MIXED>);
+    else
+        return nullptr;
+}
 }  // namespace
 
 // Loads the code object of a class's kernels (hipFuncGetAttributes) so that
@@ -64,6 +85,22 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
     const void *sym = table[cls][mixed ? 1 : 0]();
     hipFuncAttributes attr;
     hipError_t e = hipFuncGetAttributes(&attr, sym);
+    if (e == hipSuccess && kRaggedClasses[cls].R == 0) {
+        static constexpr const void *(*pools[kRaggedClassCount][2])() = {
+            {&pool_symbol<BF_MODEL, 0, false>, &pool_symbol<BF_MODEL, 0, true>},
+            {&pool_symbol<BF_MODEL, 1, false>, &pool_symbol<BF_MODEL, 1, true>},
+            {&pool_symbol<BF_MODEL, 2, false>, &pool_symbol<BF_MODEL, 2, true>},
+            {&pool_symbol<BF_MODEL, 3, false>, &pool_symbol<BF_MODEL, 3, true>},
+            {&pool_symbol<BF_MODEL, 4, false>, &pool_symbol<BF_MODEL, 4, true>},
+            {&pool_symbol<BF_MODEL, 5, false>, &pool_symbol<BF_MODEL, 5, true>},
+            {&pool_symbol<BF_MODEL, 6, false>, &pool_symbol<BF_MODEL, 6, true>},
+            {&pool_symbol<BF_MODEL, 7, false>, &pool_symbol<BF_MODEL, 7, true>},
+            {&pool_symbol<BF_MODEL, 8, false>, &pool_symbol<BF_MODEL, 8, true>},
+            {&pool_symbol<BF_MODEL, 9, false>, &pool_symbol<BF_MODEL, 9, true>},
+            {&pool_symbol<BF_MODEL, 10, false>, &pool_symbol<BF_MODEL, 10, true>},
+            {&pool_symbol<BF_MODEL, 11, false>, &pool_symbol<BF_MODEL, 11, true>}};
+        e = hipFuncGetAttributes(&attr, pools[cls][mixed ? 1 : 0]());
+    }
     if (e == hipSuccess && guess)
         e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&guess_ragged_kernel<BF_MODEL>));
     return e;

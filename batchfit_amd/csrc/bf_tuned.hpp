@@ -16,10 +16,10 @@ struct TunedRange {
 };
 constexpr int kTunedRanges = 4;
 constexpr TunedRange kTuned[4][kTunedRanges] = {
-    {{40, 8, 0}, {112, 16, 0}, {896, 8, 1}, {0xffffffffu, 16, 1}},  // gaussian
-    {{80, 16, 0}, {160, 8, 0}, {1792, 8, 1}, {0xffffffffu, 32, 1}},  // lorentzian
-    {{40, 8, 0}, {160, 16, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // pseudo_voigt
-    {{80, 8, 0}, {224, 8, 1}, {640, 64, 0}, {0xffffffffu, 64, 1}},  // two_lorentzian
+    {{56, 8, 1}, {256, 8, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // gaussian
+    {{40, 8, 1}, {256, 8, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // lorentzian
+    {{56, 8, 1}, {224, 16, 0}, {256, 8, 0}, {0xffffffffu, 8, 1}},  // pseudo_voigt
+    {{160, 8, 1}, {0xffffffffu, 64, 1}, {0xffffffffu, 64, 1}, {0xffffffffu, 64, 1}},  // two_lorentzian
 };
 
 }  // namespace bfhip

@@ -216,6 +216,22 @@ def test_streamed_equals_register_tile(hip, model, g, m):
         fields_bitwise_equal(a, b, f"{model} G={g} M={m} uncertainties={unc}")
 
 
+@pytest.mark.parametrize("model,m", [("gaussian", 256), ("pseudo_voigt", 200),
+                                     ("two_lorentzian", 160)])
+def test_streamed_equals_register_tile_large_batch(hip, model, m):
+    """A batch large enough that every wave of the streamed kernel refills
+    its pool many times (fits retire one by one): bit for bit the register
+    tile's result, from the device guess, so a skipped or repeated fit shows."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 40000, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    a = hip.fit(x, y, model, None, bounds, "hip", mapping="group8")
+    b = hip.fit(x, y, model, None, bounds, "hip", mapping="group8_streamed")
+    fields_bitwise_equal(a, b, f"{model} M={m} N=40000")
+    c = hip.fit(x, y, model, None, bounds, "hip", mapping="group64_streamed")
+    d = hip.fit(x, y, model, None, bounds, "hip", mapping="group64")
+    fields_bitwise_equal(c, d, f"{model} M={m} N=40000, 64 lanes")
+
+
 @pytest.mark.parametrize("model", list(P))
 @pytest.mark.parametrize("g,m", [(8, 700), (16, 1500), (32, 2600)])
 def test_streamed_groups_beyond_the_tile(hip, oracle_port, model, g, m):

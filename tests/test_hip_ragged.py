@@ -64,6 +64,20 @@ def test_ragged_equals_uniform_path(hip):
                 assert same(u, v), (model, m)
 
 
+@pytest.mark.parametrize("model,m", [("pseudo_voigt", 900), ("two_lorentzian", 700),
+                                     ("gaussian", 3000)])
+def test_ragged_equals_uniform_large_streamed_class(hip, model, m):
+    """A streamed class large enough that every wave refills its pool many
+    times, from the device guess: the same bytes as the uniform launch."""
+    from batchfit_amd.ragged import fit_ragged
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 12000, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    a = hip.fit(x, y, model, None, bounds, "hip")
+    b = fit_ragged([(x, y, model, None, bounds)])[0]
+    for u, v in zip(a, b):
+        assert same(u, v), (model, m)
+
+
 def test_ragged_uncertainties_and_failures(hip, oracle_port):
     from batchfit_amd.ragged import fit_ragged
     x, y, p0, bounds, _ = datasets.comparison_set("gaussian", 20, 129)
