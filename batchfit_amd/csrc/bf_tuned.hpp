@@ -14,12 +14,12 @@ struct TunedRange {
     int lanes;
     int streamed;  // 1: points read from memory on every pass
 };
-constexpr int kTunedRanges = 4;
+constexpr int kTunedRanges = 5;
 constexpr TunedRange kTuned[4][kTunedRanges] = {
-    {{56, 8, 1}, {256, 8, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // gaussian
-    {{40, 8, 1}, {256, 8, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // lorentzian
-    {{56, 8, 1}, {224, 16, 0}, {256, 8, 0}, {0xffffffffu, 8, 1}},  // pseudo_voigt
-    {{160, 8, 1}, {0xffffffffu, 64, 1}, {0xffffffffu, 64, 1}, {0xffffffffu, 64, 1}},  // two_lorentzian
+    {{56, 8, 1}, {256, 8, 0}, {1792, 8, 1}, {0xffffffffu, 32, 1}, {0xffffffffu, 32, 1}},  // gaussian
+    {{40, 8, 1}, {256, 8, 0}, {512, 16, 0}, {1280, 8, 1}, {0xffffffffu, 16, 1}},  // lorentzian
+    {{80, 8, 1}, {224, 16, 0}, {256, 8, 0}, {0xffffffffu, 8, 1}, {0xffffffffu, 8, 1}},  // pseudo_voigt
+    {{80, 8, 1}, {2560, 16, 1}, {0xffffffffu, 32, 1}, {0xffffffffu, 32, 1}, {0xffffffffu, 32, 1}},  // two_lorentzian
 };
 
 }  // namespace bfhip
