@@ -21,14 +21,20 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
         constexpr int per_block = 256 / G;
         const uint32_t blocks = (args.n + per_block - 1) / per_block;
         if constexpr (R == 0 && G > 1) {
-            if (args.pool && !args.cov) {
+            if (args.pool) {
                 // Persistent waves that retire fits one by one (solve_pool).
                 // One refill per wave to start with, at most 4096 waves (more
-                // than the device holds at once).
+                // than the device holds at once); uncertainties follow in a
+                // second launch, one group per fit.
                 const uint32_t waves = (args.n + pool_refill(G) - 1) / pool_refill(G);
                 const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
                 hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED>), dim3(pool_blocks),
                                    dim3(256), 0, stream, args);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess || !args.cov)
+                    return e;
+                hipLaunchKernelGGL((lm_cov_kernel<MODEL, G, MIXED>), dim3(blocks), dim3(256), 0,
+                                   stream, args);
                 return hipGetLastError();
             }
         }

@@ -1069,6 +1069,26 @@ __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, 0)) lm_pool_kernel
     solve_pool<MODEL, G, COMP, S>(lane % G, src);
 }
 
+// Uncertainties after a retiring launch: one group per fit, from the fit's
+// written parameters, status and χ² (the values solve_fit hands to
+// covariance), over the streamed points of the same group size.
+template <int MODEL, int G, bool MIXED>
+__global__ void __launch_bounds__(256) lm_cov_kernel(FitArgs a) {
+    using S = typename std::conditional<MIXED, double, float>::type;
+    constexpr int P = Params<MODEL>::P;
+    const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int gl = int(threadIdx.x % G);
+    if (fit >= a.n)
+        return;
+    float p[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        p[k] = a.params[size_t(fit) * P + k];
+    Points<G, 0, stream_chunk(MODEL)> pts;
+    pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
+    covariance<MODEL, G, 0, S>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit]);
+}
+
 template <int MODEL, int G, int R, bool COMP, bool MIXED>
 __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, R)) lm_kernel(FitArgs a) {
     constexpr int per_block = 256 / G;

@@ -206,6 +206,29 @@ __global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, 0))
     solve_pool<MODEL, G, MIXED, S>(lane % G, src);
 }
 
+// Uncertainties of a streamed class after its retiring launch: one group per
+// fit, from the fit's written parameters, status and χ².
+template <int MODEL, int G, bool MIXED>
+__global__ void __launch_bounds__(256) lm_ragged_cov_kernel(RaggedArgs r) {
+    using S = typename std::conditional<MIXED, double, float>::type;
+    constexpr int P = Params<MODEL>::P;
+    const uint32_t task = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int gl = int(threadIdx.x % G);
+    if (task >= r.count)
+        return;
+    bf_fit_desc d = r.fits[r.order[task]];
+    if constexpr (G == 64)
+        d = wave_uniform(d);
+    const FitArgs a = ragged_fit_args<G>(r, d, MODEL, P);
+    float p[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        p[k] = a.params[k];
+    Points<G, 0, stream_chunk(MODEL)> pts;
+    pts.load(a.x, a.y, 1, gl, int(a.m));
+    covariance<MODEL, G, 0, S>(a, pts, 0, gl, p, a.status[0], a.chi2[0]);
+}
+
 // Data-only guess of one class of a ragged batch, one wave per fit; LDS sized
 // by the host for the class's largest M.
 template <int MODEL> __global__ void __launch_bounds__(64) guess_ragged_kernel(

@@ -18,15 +18,20 @@ template <int MODEL, int CLS, bool MIXED>
 hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream) {
     constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
     if constexpr (R == 0) {
-        if (!args.cov) {
-            // Streamed class, no uncertainties: retiring groups; one refill
-            // per wave to start with, at most 4096 waves.
-            const uint32_t waves = (args.count + pool_refill(G) - 1) / pool_refill(G);
-            const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
-            hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED>), dim3(pool_blocks),
-                               dim3(256), 0, stream, args);
-            return hipGetLastError();
-        }
+        // Streamed class: retiring groups (one refill per wave to start with,
+        // at most 4096 waves), then uncertainties one group per fit.
+        const uint32_t waves = (args.count + pool_refill(G) - 1) / pool_refill(G);
+        const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
+        hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED>), dim3(pool_blocks),
+                           dim3(256), 0, stream, args);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !args.cov)
+            return e;
+        constexpr uint32_t per_block = 256 / G;
+        hipLaunchKernelGGL((lm_ragged_cov_kernel<MODEL, G, MIXED>),
+                           dim3((args.count + per_block - 1) / per_block), dim3(256), 0, stream,
+                           args);
+        return hipGetLastError();
     }
     hipLaunchKernelGGL((lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>), dim3(blocks), dim3(256), 0,
                        stream, args);
@@ -59,6 +64,14 @@ template <int MODEL, int CLS, bool MIXED> const void *pool_symbol() {
     constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
     if constexpr (R == 0)
         return reinterpret_cast<const void *>(&lm_ragged_pool_kernel<MODEL, G, MIXED>);
+    else
+        return nullptr;
+}
+// Its uncertainty kernel (null for a tile class).
+template <int MODEL, int CLS, bool MIXED> const void *cov_symbol() {
+    constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    if constexpr (R == 0)
+        return reinterpret_cast<const void *>(&lm_ragged_cov_kernel<MODEL, G, MIXED>);
     else
         return nullptr;
 }
@@ -100,6 +113,21 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
             {&pool_symbol<BF_MODEL, 10, false>, &pool_symbol<BF_MODEL, 10, true>},
             {&pool_symbol<BF_MODEL, 11, false>, &pool_symbol<BF_MODEL, 11, true>}};
         e = hipFuncGetAttributes(&attr, pools[cls][mixed ? 1 : 0]());
+        static constexpr const void *(*covs[kRaggedClassCount][2])() = {
+            {&cov_symbol<BF_MODEL, 0, false>, &cov_symbol<BF_MODEL, 0, true>},
+            {&cov_symbol<BF_MODEL, 1, false>, &cov_symbol<BF_MODEL, 1, true>},
+            {&cov_symbol<BF_MODEL, 2, false>, &cov_symbol<BF_MODEL, 2, true>},
+            {&cov_symbol<BF_MODEL, 3, false>, &cov_symbol<BF_MODEL, 3, true>},
+            {&cov_symbol<BF_MODEL, 4, false>, &cov_symbol<BF_MODEL, 4, true>},
+            {&cov_symbol<BF_MODEL, 5, false>, &cov_symbol<BF_MODEL, 5, true>},
+            {&cov_symbol<BF_MODEL, 6, false>, &cov_symbol<BF_MODEL, 6, true>},
+            {&cov_symbol<BF_MODEL, 7, false>, &cov_symbol<BF_MODEL, 7, true>},
+            {&cov_symbol<BF_MODEL, 8, false>, &cov_symbol<BF_MODEL, 8, true>},
+            {&cov_symbol<BF_MODEL, 9, false>, &cov_symbol<BF_MODEL, 9, true>},
+            {&cov_symbol<BF_MODEL, 10, false>, &cov_symbol<BF_MODEL, 10, true>},
+            {&cov_symbol<BF_MODEL, 11, false>, &cov_symbol<BF_MODEL, 11, true>}};
+        if (e == hipSuccess)
+            e = hipFuncGetAttributes(&attr, covs[cls][mixed ? 1 : 0]());
     }
     if (e == hipSuccess && guess)
         e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&guess_ragged_kernel<BF_MODEL>));

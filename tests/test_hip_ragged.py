@@ -76,6 +76,11 @@ def test_ragged_equals_uniform_large_streamed_class(hip, model, m):
     b = fit_ragged([(x, y, model, None, bounds)])[0]
     for u, v in zip(a, b):
         assert same(u, v), (model, m)
+    # With uncertainties (a second launch after the retiring one, both paths).
+    a = hip.fit(x[:], y[:3000], model, None, bounds, "hip", uncertainties=True)
+    b = fit_ragged([(x, y[:3000], model, None, bounds)], uncertainties=True)[0]
+    for name, u, v in zip(a._fields, a, b):
+        assert same(u, v), (model, m, name)
 
 
 def test_ragged_uncertainties_and_failures(hip, oracle_port):
