@@ -28,8 +28,12 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
                 // second launch, one group per fit.
                 const uint32_t waves = (args.n + pool_refill(G) - 1) / pool_refill(G);
                 const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
-                hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED>), dim3(pool_blocks),
-                                   dim3(256), 0, stream, args);
+                if (args.flat_check)
+                    hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, true>),
+                                       dim3(pool_blocks), dim3(256), 0, stream, args);
+                else
+                    hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, false>),
+                                       dim3(pool_blocks), dim3(256), 0, stream, args);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess || !args.cov)
                     return e;

@@ -55,6 +55,7 @@ struct FitArgs {
     uint32_t *pool;      // work counter of the retiring streamed kernel (zeroed), or null
     int32_t options_ok;  // λ and tolerances valid, bounds ordered/non-NaN/non-empty-side
     int32_t free_shape;  // cpu/solver.hpp:70-82
+    int32_t flat_check;  // some fit may have flat observations (its p0 is not a device guess)
     float width_factor;  // cpu/solver.hpp:86-88
 };
 
@@ -141,7 +142,7 @@ __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
 }
 
 // Per-lane partial accumulation. Plain: one FMA per term and point.
-// Compensated: FMA into a block partial over 16 points of the lane (8 for the
+// Compensated: FMA into a block partial over 32 points of the lane (8 for the
 // two-peak model), then the block partial is added into a Kahan pair (sum,
 // carry) (numerics.hpp:12-18), so the compensation costs 4 VALU per term per
 // block instead of per point. A lane's partial is (sum − carry).
@@ -223,7 +224,7 @@ template <int G, int R, int CHUNK = 16> struct Points {
     // f(x, y) for every point this lane owns, then flush() after every block
     // of kBlock points. Slots below m/G are full for every lane of the group
     // (a scalar branch); only slot m/G is partial.
-    static constexpr int kBlock = CHUNK < 16 ? CHUNK : 16;
+    static constexpr int kBlock = CHUNK < 16 ? CHUNK : 32;
     static constexpr int kChunk = CHUNK;
     template <typename F, typename Flush>
     __device__ __forceinline__ void each(F &&f, Flush &&flush) const {
@@ -252,12 +253,14 @@ template <int G, int R, int CHUNK = 16> struct Points {
             }
         } else {
             int j = gl;
+            int k = 0;  // points since the last flush
             if constexpr (G > 1) {
                 // Whole chunks of kChunk points per lane: all their loads are
                 // issued before the first point is evaluated, so one memory
                 // latency is paid per chunk instead of per point. Points and
                 // flushes come in the same order as in the per-point loop.
-                static_assert(kChunk % kBlock == 0, "chunks hold whole blocks");
+                static_assert(kChunk % kBlock == 0 || kBlock % kChunk == 0,
+                              "blocks and chunks nest");
                 for (; j + G * (kChunk - 1) < m; j += G * kChunk) {
                     float xc[kChunk], yc[kChunk];
 #pragma unroll
@@ -268,12 +271,19 @@ template <int G, int R, int CHUNK = 16> struct Points {
 #pragma unroll
                     for (int c = 0; c < kChunk; ++c) {
                         f(xc[c], yc[c]);
-                        if (c % kBlock == kBlock - 1)
+                        if constexpr (kChunk >= kBlock)
+                            if (c % kBlock == kBlock - 1)
+                                flush();
+                    }
+                    if constexpr (kBlock > kChunk) {
+                        k += kChunk;
+                        if (k == kBlock) {
                             flush();
+                            k = 0;
+                        }
                     }
                 }
             }
-            int k = 0;
             for (; j < m; j += G) {
                 f(x[j], y[j * ys_pt]);
                 if (++k == kBlock) {
@@ -763,7 +773,12 @@ __device__ __forceinline__ void solve_fit(const FitArgs &a, uint32_t fit, int gl
 // — the current fit's output row in args().
 constexpr uint32_t kNoFit = 0xffffffffu;
 
-template <int MODEL, int G, bool COMP, typename S, typename Source>
+// FLAT: test the degenerate case (flat observations, cpu/solver.hpp:237-245)
+// from each first pass's observation range. A fit that starts from a
+// successful device guess cannot have flat observations (the guess needs a
+// smoothed peak above three noise scales), so launches whose every initial
+// point is a guess leave the range out of their passes.
+template <int MODEL, int G, bool COMP, bool FLAT, typename S, typename Source>
 __device__ __forceinline__ void solve_pool(int gl, Source &src) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     constexpr float eps = 1.1920928955078125e-7f;
@@ -908,7 +923,7 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
         S terms[Params<MODEL>::TERMS];
         bool pass_ok;
         float yrange[2];
-        fused_pass<MODEL, G, 0, COMP, S>(due, trial, terms, pass_ok, yrange);
+        fused_pass<MODEL, G, 0, COMP, S>(due, trial, terms, pass_ok, FLAT ? yrange : nullptr);
         // 4. After the pass (no early exits: every path reaches the next trip's
         //    test the same way).
         if (fit != kNoFit) {
@@ -932,7 +947,7 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
                         p[k] = trial[k];
                     chi = pass_chi;
                     it = 0;
-                    if (yrange[0] == yrange[1] && src.grid().varied_x && c.free_shape) {
+                    if (FLAT && yrange[0] == yrange[1] && src.grid().varied_x && c.free_shape) {
                         finish(ST_DEGENERATE);
                     } else {
 #pragma unroll
@@ -1061,12 +1076,12 @@ constexpr int lm_min_waves(int model, int g, int r) {
 
 // Streamed groups with retiring fits (no uncertainties): persistent waves over
 // the batch counter a.pool.
-template <int MODEL, int G, bool COMP, bool MIXED>
+template <int MODEL, int G, bool COMP, bool MIXED, bool FLAT>
 __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, 0)) lm_pool_kernel(FitArgs a) {
     using S = typename std::conditional<MIXED, double, float>::type;
     const int lane = int(threadIdx.x & 63);
     BatchPool<G> src(a, lane, lane % G);
-    solve_pool<MODEL, G, COMP, S>(lane % G, src);
+    solve_pool<MODEL, G, COMP, FLAT, S>(lane % G, src);
 }
 
 // Uncertainties after a retiring launch: one group per fit, from the fit's

@@ -27,6 +27,7 @@ struct RaggedArgs {
     const uint32_t *order;      // this class's fits, longest first
     uint32_t count;             // fits in this class
     uint32_t *cursor;           // this class's work-queue counter (reset before the launch)
+    uint32_t flat_check;        // some fit of the class starts from a given p0
     const float *x, *y, *p0;    // p0 rows of 8, or the guess buffer
     const float *guess;         // rows of 8 written by the guess pass
     float *params;              // rows of 8
@@ -62,6 +63,7 @@ __device__ __forceinline__ FitArgs ragged_fit_args(const RaggedArgs &r, const bf
         c = wave_uniform(c);
     FitArgs a;
     a.pool = nullptr;
+    a.flat_check = 1;
     a.x = r.x + r.grids[d.grid].x_offset;
     a.y = r.y + d.y_offset;
     a.ys_fit = 0;
@@ -197,13 +199,13 @@ template <int MODEL, int G> struct RaggedPool : PoolQueue<G> {
 
 // A streamed class without uncertainties: persistent waves whose groups
 // retire fits one by one (solve_pool) from the class's counter.
-template <int MODEL, int G, bool MIXED>
+template <int MODEL, int G, bool MIXED, bool FLAT>
 __global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, 0))
     lm_ragged_pool_kernel(RaggedArgs r) {
     using S = typename std::conditional<MIXED, double, float>::type;
     const int lane = int(threadIdx.x & 63);
     RaggedPool<MODEL, G> src(r, lane, lane % G);
-    solve_pool<MODEL, G, MIXED, S>(lane % G, src);
+    solve_pool<MODEL, G, MIXED, FLAT, S>(lane % G, src);
 }
 
 // Uncertainties of a streamed class after its retiring launch: one group per

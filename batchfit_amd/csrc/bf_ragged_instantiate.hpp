@@ -22,8 +22,12 @@ hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream
         // at most 4096 waves), then uncertainties one group per fit.
         const uint32_t waves = (args.count + pool_refill(G) - 1) / pool_refill(G);
         const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
-        hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED>), dim3(pool_blocks),
-                           dim3(256), 0, stream, args);
+        if (args.flat_check)
+            hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, true>), dim3(pool_blocks),
+                               dim3(256), 0, stream, args);
+        else
+            hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, false>), dim3(pool_blocks),
+                               dim3(256), 0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || !args.cov)
             return e;
@@ -63,7 +67,14 @@ template <int MODEL, int CLS, bool MIXED> const void *ragged_symbol() {
 template <int MODEL, int CLS, bool MIXED> const void *pool_symbol() {
     constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
     if constexpr (R == 0)
-        return reinterpret_cast<const void *>(&lm_ragged_pool_kernel<MODEL, G, MIXED>);
+        return reinterpret_cast<const void *>(&lm_ragged_pool_kernel<MODEL, G, MIXED, true>);
+    else
+        return nullptr;
+}
+template <int MODEL, int CLS, bool MIXED> const void *pool_symbol_guessed() {
+    constexpr int G = kRaggedClasses[CLS].G, R = kRaggedClasses[CLS].R;
+    if constexpr (R == 0)
+        return reinterpret_cast<const void *>(&lm_ragged_pool_kernel<MODEL, G, MIXED, false>);
     else
         return nullptr;
 }
@@ -113,6 +124,21 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
             {&pool_symbol<BF_MODEL, 10, false>, &pool_symbol<BF_MODEL, 10, true>},
             {&pool_symbol<BF_MODEL, 11, false>, &pool_symbol<BF_MODEL, 11, true>}};
         e = hipFuncGetAttributes(&attr, pools[cls][mixed ? 1 : 0]());
+        static constexpr const void *(*guessed[kRaggedClassCount][2])() = {
+            {&pool_symbol_guessed<BF_MODEL, 0, false>, &pool_symbol_guessed<BF_MODEL, 0, true>},
+            {&pool_symbol_guessed<BF_MODEL, 1, false>, &pool_symbol_guessed<BF_MODEL, 1, true>},
+            {&pool_symbol_guessed<BF_MODEL, 2, false>, &pool_symbol_guessed<BF_MODEL, 2, true>},
+            {&pool_symbol_guessed<BF_MODEL, 3, false>, &pool_symbol_guessed<BF_MODEL, 3, true>},
+            {&pool_symbol_guessed<BF_MODEL, 4, false>, &pool_symbol_guessed<BF_MODEL, 4, true>},
+            {&pool_symbol_guessed<BF_MODEL, 5, false>, &pool_symbol_guessed<BF_MODEL, 5, true>},
+            {&pool_symbol_guessed<BF_MODEL, 6, false>, &pool_symbol_guessed<BF_MODEL, 6, true>},
+            {&pool_symbol_guessed<BF_MODEL, 7, false>, &pool_symbol_guessed<BF_MODEL, 7, true>},
+            {&pool_symbol_guessed<BF_MODEL, 8, false>, &pool_symbol_guessed<BF_MODEL, 8, true>},
+            {&pool_symbol_guessed<BF_MODEL, 9, false>, &pool_symbol_guessed<BF_MODEL, 9, true>},
+            {&pool_symbol_guessed<BF_MODEL, 10, false>, &pool_symbol_guessed<BF_MODEL, 10, true>},
+            {&pool_symbol_guessed<BF_MODEL, 11, false>, &pool_symbol_guessed<BF_MODEL, 11, true>}};
+        if (e == hipSuccess)
+            e = hipFuncGetAttributes(&attr, guessed[cls][mixed ? 1 : 0]());
         static constexpr const void *(*covs[kRaggedClassCount][2])() = {
             {&cov_symbol<BF_MODEL, 0, false>, &cov_symbol<BF_MODEL, 0, true>},
             {&cov_symbol<BF_MODEL, 1, false>, &cov_symbol<BF_MODEL, 1, true>},

@@ -193,6 +193,7 @@ hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice 
     a.order = dev.order + r.offset;
     a.count = r.count;
     a.cursor = dev.cursors + i;
+    a.flat_check = r.guesses < r.count;
     a.x = io.x;
     a.y = io.y;
     a.p0 = io.p0;

@@ -281,6 +281,7 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     a.ustatus = us;
     a.grid = grid;
     a.pool = pool;
+    a.flat_check = p0 != nullptr;
     a.n = n;
     a.m = m;
     a.max_iterations = cfg->max_iterations;
