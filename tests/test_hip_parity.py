@@ -216,6 +216,20 @@ def test_streamed_equals_register_tile(hip, model, g, m):
         fields_bitwise_equal(a, b, f"{model} G={g} M={m} uncertainties={unc}")
 
 
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 15, 16, 17, 33, 100])
+def test_streamed_pool_batch_sizes(hip, n):
+    """Batches that end inside a wave's pool or a group: the retiring kernel
+    with and without the flat-observation test (given p0 / device guess)
+    gives the register tile's bytes."""
+    for model, m in (("pseudo_voigt", 200), ("two_lorentzian", 100)):
+        x, y, p0, bounds, _ = datasets.comparison_set(model, max(n, 8), m)
+        x, y = x.astype(np.float32), y[:n].astype(np.float32)
+        for start in (p0, None):
+            a = hip.fit(x, y, model, start, bounds, "hip", mapping="group8")
+            b = hip.fit(x, y, model, start, bounds, "hip", mapping="group8_streamed")
+            fields_bitwise_equal(a, b, f"{model} M={m} N={n} p0={'given' if start is not None else 'guess'}")
+
+
 @pytest.mark.parametrize("model,m", [("gaussian", 256), ("pseudo_voigt", 200),
                                      ("two_lorentzian", 160)])
 def test_streamed_equals_register_tile_large_batch(hip, model, m):
