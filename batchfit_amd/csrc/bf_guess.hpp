@@ -183,7 +183,11 @@ __device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p))
 // (initial_guess.hpp:25-30: the middle order statistic, and for an even count
 // the midpoint of it and the largest value below it). The keys stay in
 // registers; the k-th smallest key is found bit by bit from the top, each
-// step counting the keys at or below a bound with one ballot per slot.
+// step counting the keys at or below a bound with one ballot per slot. The
+// search keeps the interval [t, t + 2^b) that holds the k-th key with the
+// count of keys below it (lo) and at or below its top (hi); once the
+// interval holds a single key, that key is the answer and the remaining bits
+// are read from it instead of searched.
 template <int SLOTS, typename F>
 __device__ float select_median(int count, int lane, F &&get) {
     uint32_t key[SLOTS];
@@ -194,26 +198,40 @@ __device__ float select_median(int count, int lane, F &&get) {
     }
     const int k = count / 2;
     uint32_t t = 0;
+    int lo = 0, hi = 64 * SLOTS;  // lo ≤ k < hi throughout
     for (int b = 31; b >= 0; --b) {
-        const uint32_t low = t | ((1u << b) - 1u);
+        const uint32_t span = (1u << b) - 1u;
         int c = 0;
 #pragma unroll
         for (int q = 0; q < SLOTS; ++q)
-            c += wave_count(key[q] <= low);
-        if (c < k + 1)
+            c += wave_count(key[q] <= (t | span));
+        if (c < k + 1) {
             t |= 1u << b;
+            lo = c;
+        } else {
+            hi = c;
+        }
+        if (hi - lo == 1) {  // one key in [t, t + span]: lo = k, so it is the k-th
+            uint32_t found = t;
+#pragma unroll
+            for (int q = 0; q < SLOTS; ++q) {
+                const uint64_t in = __ballot(key[q] - t <= span);
+                if (in)
+                    found = uint32_t(__builtin_amdgcn_readlane(int(key[q]), __builtin_ctzll(in)));
+            }
+            t = found;
+            break;
+        }
     }
     if (count % 2)
         return key_float(t);
-    int below = 0;
+    // lo counts the keys below t whether the search ended early or not.
     uint32_t under = 0;
 #pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-        below += wave_count(key[q] < t);
+    for (int q = 0; q < SLOTS; ++q)
         under = (key[q] < t && key[q] > under) ? key[q] : under;
-    }
     uint32_t lower = t;
-    if (below >= k) {  // the k smallest are all below t: the largest of them
+    if (lo >= k) {  // the k smallest are all below t: the largest of them
         for (int o = 32; o > 0; o >>= 1) {
             const uint32_t w = uint32_t(__shfl_xor(int(under), o, 64));
             under = w > under ? w : under;
