@@ -103,6 +103,26 @@ __device__ __forceinline__ int wave_or_i(int v) {
         v |= __shfl_xor(v, o, 64);
     return v;
 }
+// Wave-wide minimum and maximum of unsigned values, in every lane (the
+// all-reduce steps of xlane), made wave-uniform.
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+    v = min(v, uint32_t(xlane<1>(int(v))));
+    v = min(v, uint32_t(xlane<2>(int(v))));
+    v = min(v, uint32_t(xlane<4>(int(v))));
+    v = min(v, uint32_t(xlane<8>(int(v))));
+    v = min(v, uint32_t(xlane<16>(int(v))));
+    v = min(v, uint32_t(xlane<32>(int(v))));
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+}
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+    v = max(v, uint32_t(xlane<1>(int(v))));
+    v = max(v, uint32_t(xlane<2>(int(v))));
+    v = max(v, uint32_t(xlane<4>(int(v))));
+    v = max(v, uint32_t(xlane<8>(int(v))));
+    v = max(v, uint32_t(xlane<16>(int(v))));
+    v = max(v, uint32_t(xlane<32>(int(v))));
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+}
 // Exact wave minimum of floats with no NaN (DPP and swizzle steps).
 __device__ __forceinline__ float wave_min_fx(float v) {
     v = fminf(v, __int_as_float(xor_lane<1>(__float_as_int(v))));
@@ -182,24 +202,33 @@ __device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p))
 // Median of count ≤ 64·SLOTS finite values get(0..count) by exact selection
 // (initial_guess.hpp:25-30: the middle order statistic, and for an even count
 // the midpoint of it and the largest value below it). The keys stay in
-// registers; the k-th smallest key is found bit by bit from the top, each
-// step counting the keys at or below a bound with one ballot per slot. The
-// search keeps the interval [t, t + 2^b) that holds the k-th key with the
-// count of keys below it (lo) and at or below its top (hi); once the
-// interval holds a single key, that key is the answer and the remaining bits
-// are read from it instead of searched.
+// registers; the k-th smallest key is found bit by bit, each step counting
+// the keys at or below a bound with one ballot per slot. The search keeps
+// the interval [t, t + 2^b) that holds the k-th key with the count of keys
+// below it (lo) and an upper bound on the count at or below its top (hi,
+// exact once the search has narrowed from above). It starts below the bits
+// that the smallest and largest keys share, and once the interval holds a
+// single key, that key is the answer and the remaining bits are read from it
+// instead of searched.
 template <int SLOTS, typename F>
 __device__ float select_median(int count, int lane, F &&get) {
     uint32_t key[SLOTS];
+    uint32_t kmin = 0xffffffffu, kmax = 0;
 #pragma unroll
     for (int q = 0; q < SLOTS; ++q) {
         const int e = q * 64 + lane;
         key[q] = e < count ? float_key(get(e)) : 0xffffffffu;  // above every finite key
+        kmin = min(kmin, key[q]);
+        kmax = max(kmax, e < count ? key[q] : 0u);
     }
+    kmin = wave_min_u(kmin);
+    kmax = wave_max_u(kmax);
+    const uint32_t diff = kmin ^ kmax;
+    const int top = diff ? 31 - __builtin_clz(diff) : -1;  // highest bit that differs
     const int k = count / 2;
-    uint32_t t = 0;
-    int lo = 0, hi = 64 * SLOTS;  // lo ≤ k < hi throughout
-    for (int b = 31; b >= 0; --b) {
+    uint32_t t = top >= 0 ? kmin & ~((2u << top) - 1u) : kmin;  // 2u << 31 is 0
+    int lo = 0, hi = 64 * SLOTS;  // lo ≤ k < exact count at or below the top
+    for (int b = top; b >= 0; --b) {
         const uint32_t span = (1u << b) - 1u;
         int c = 0;
 #pragma unroll
@@ -231,13 +260,8 @@ __device__ float select_median(int count, int lane, F &&get) {
     for (int q = 0; q < SLOTS; ++q)
         under = (key[q] < t && key[q] > under) ? key[q] : under;
     uint32_t lower = t;
-    if (lo >= k) {  // the k smallest are all below t: the largest of them
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint32_t w = uint32_t(__shfl_xor(int(under), o, 64));
-            under = w > under ? w : under;
-        }
-        lower = under;
-    }
+    if (lo >= k)  // the k smallest are all below t: the largest of them
+        lower = wave_max_u(under);
     return std_midpoint(key_float(lower), key_float(t));
 }
 
