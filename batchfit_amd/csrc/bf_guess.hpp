@@ -103,16 +103,16 @@ __device__ __forceinline__ int wave_or_i(int v) {
         v |= __shfl_xor(v, o, 64);
     return v;
 }
-// Wave-wide minimum and maximum of unsigned values, in every lane (the
-// all-reduce steps of xlane), made wave-uniform.
+// Wave-wide minimum and maximum of unsigned values as scalars: the
+// all-reduce steps of xlane within each 32-lane half, then lanes 0 and 32.
 __device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
     v = min(v, uint32_t(xlane<1>(int(v))));
     v = min(v, uint32_t(xlane<2>(int(v))));
     v = min(v, uint32_t(xlane<4>(int(v))));
     v = min(v, uint32_t(xlane<8>(int(v))));
     v = min(v, uint32_t(xlane<16>(int(v))));
-    v = min(v, uint32_t(xlane<32>(int(v))));
-    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+    return min(uint32_t(__builtin_amdgcn_readlane(int(v), 0)),
+               uint32_t(__builtin_amdgcn_readlane(int(v), 32)));
 }
 __device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
     v = max(v, uint32_t(xlane<1>(int(v))));
@@ -120,8 +120,8 @@ __device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
     v = max(v, uint32_t(xlane<4>(int(v))));
     v = max(v, uint32_t(xlane<8>(int(v))));
     v = max(v, uint32_t(xlane<16>(int(v))));
-    v = max(v, uint32_t(xlane<32>(int(v))));
-    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+    return max(uint32_t(__builtin_amdgcn_readlane(int(v), 0)),
+               uint32_t(__builtin_amdgcn_readlane(int(v), 32)));
 }
 // Exact wave minimum of floats with no NaN (DPP and swizzle steps).
 __device__ __forceinline__ float wave_min_fx(float v) {
@@ -301,6 +301,29 @@ __device__ __forceinline__ float smooth_at(const float *s, int m, int i) {
     return 0.25f * s[i - 1] + 0.5f * s[i] + 0.25f * s[i + 1];
 }
 
+// Smoothed value at 1 ≤ j < m as smooth_at (the raw value at the last
+// point); j ≥ m gives 0, a value no caller uses.
+__device__ __forceinline__ float smooth_in(const float *s, int m, int j) {
+    if (j + 1 < m)
+        return 0.25f * s[j - 1] + 0.5f * s[j] + 0.25f * s[j + 1];
+    return j < m ? s[j] : 0.0f;
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+// The value of lane − 1, with `first` in lane 0 (DPP wave_shr:1), and of
+// lane + 1, with `last` in lane 63 (DPP wave_shl:1). All 64 lanes active.
+__device__ __forceinline__ float from_lane_below(float v, float first) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, first),
+                                                                 __builtin_bit_cast(int, v), 0x138,
+                                                                 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_lane_above(float v, float last) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, last),
+                                                                 __builtin_bit_cast(int, v), 0x130,
+                                                                 0xF, 0xF, false));
+}
+
 // initial_guess.hpp:61-109 with both half-height walks as wave reductions.
 __device__ GuessPeak measure_peak_dev(const float *x, const float *s, int m, int index, int begin,
                                       int end, bool mirror, int lane) {
@@ -309,15 +332,25 @@ __device__ GuessPeak measure_peak_dev(const float *x, const float *s, int m, int
     if (!(r.amplitude > 0.0f))
         return r;
     const float half = r.amplitude / 2.0f;
+    // The nearest point at or below half on each side: 64-point steps
+    // outwards from the peak, ending at the first step that holds one.
     int left_c = -1, right_c = 0x7fffffff;
-    for (int j = begin + lane; j < index; j += 64)
-        if (s[j] <= half)
-            left_c = j;
-    for (int j = index + 1 + lane; j <= end; j += 64)
-        if (s[j] <= half && j < right_c)
-            right_c = j;
-    left_c = wave_max_i(left_c);
-    right_c = wave_min_i(right_c);
+    for (int top = index; top > begin; top -= 64) {  // the step [top - 64, top)
+        const int j = top - 64 + lane;
+        const uint64_t below = __ballot(j >= begin && s[j] <= half);
+        if (below) {
+            left_c = top - 1 - __builtin_clzll(below);
+            break;
+        }
+    }
+    for (int bottom = index + 1; bottom <= end; bottom += 64) {  // [bottom, bottom + 64)
+        const int j = bottom + lane;
+        const uint64_t below = __ballot(j <= end && s[j] <= half);
+        if (below) {
+            right_c = bottom + __builtin_ctzll(below);
+            break;
+        }
+    }
     const int left = left_c >= 0 ? left_c : begin;
     const int right = right_c != 0x7fffffff ? right_c : end;
     const bool has_left = s[left] <= half, has_right = s[right] <= half;
@@ -471,15 +504,26 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
     wave_sync();
     if (bad || !finite_f(noise))
         return finish();
-    // Strongest smoothed local maximum (initial_guess.hpp:170-182).
+    // Strongest smoothed local maximum (initial_guess.hpp:170-182). Each
+    // smoothed value is formed once, in the lane of its point; a point's
+    // neighbours come from the adjacent lanes, and across a step's edges
+    // from the previous and next steps.
     float best = 0.0f;
     int first = 0x7fffffff;
-#pragma unroll 4
-    for (int j = 1 + lane; j + 1 < m; j += 64) {
-        const float v = smooth_at(s, m, j);
-        if (v > best && v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1)) {
-            best = v;
-            first = j;
+    {
+        float cur = smooth_in(s, m, 1 + lane);
+        float before = s[0];  // the smoothed value at 0 is the raw one
+        for (int base = 1; base + 1 < m; base += 64) {
+            const int j = base + lane;
+            const float next = smooth_in(s, m, j + 64);
+            const float left = from_lane_below(cur, before);
+            const float right = from_lane_above(cur, lane_value(next, 0));
+            if (j + 1 < m && cur > best && cur > left && cur >= right) {
+                best = cur;
+                first = j;
+            }
+            before = lane_value(cur, 63);
+            cur = next;
         }
     }
     wave_argmax(best, first);
@@ -530,17 +574,21 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
         float pmin = s[0];
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
+        float cur = smooth_in(s, m, 1 + lane);
+        float before = s[0];
         for (int base = 1; base + 1 < m; base += 64) {
             const int j = base + lane;
-            float v = 0.0f;
+            const float v = cur;  // the smoothed value wherever j + 1 < m
+            const float next = smooth_in(s, m, j + 64);
+            const float left = from_lane_below(cur, before);
+            const float right = from_lane_above(cur, lane_value(next, 0));
+            before = lane_value(cur, 63);
+            cur = next;
             bool cand = false;
-            if (j + 1 < m)
-                v = smooth_at(s, m, j);
             pmin = fminf(pmin, wave_min_fx(j + 1 < m ? v : __uint_as_float(0x7f800000u)));
             const float right_min = sfx[(base - 1) / 64];
             if (j + 1 < m) {
-                cand = v > v_best && v - pmin > t && v - right_min > t &&
-                       v > smooth_at(s, m, j - 1) && v >= smooth_at(s, m, j + 1);
+                cand = v > v_best && v - pmin > t && v - right_min > t && v > left && v >= right;
                 if (cand) {
                     const int idx = raw_peak_dev(s, j);
                     const int bins = idx > first ? idx - first : first - idx;
