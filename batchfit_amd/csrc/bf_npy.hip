@@ -8,8 +8,9 @@
 // point-major on the host, submit, and hand each finished chunk to an ordered
 // consumer, with at most three chunks in flight.
 //
-// Here the host only moves bytes: each chunk's window of the file is mapped
-// and copied (by several threads) into a pinned slot, sent to HBM as it is
+// Here the host only moves bytes: each chunk's window of the file is read
+// by several threads (pread, no page mapping) straight into a pinned slot,
+// sent to HBM as it is
 // (float64 stays float64 over PCIe and is converted by a kernel), and fitted
 // by the uniform device path on the slot's own stream — the data-only guess
 // runs on the device and y stays fit-major, so there is no host transpose.
@@ -179,9 +180,10 @@ NpyHeader read_header(const std::string &path) {
     return h;
 }
 
-// A read-only file whose windows are mapped one chunk at a time
-// (src/cpu/mmap.cpp:91-145): page-aligned, sequential advice, read-ahead of
-// the next window.
+// A read-only file read one window at a time: y windows by parallel pread
+// into pinned memory, small windows (grid, p0) mapped as in
+// src/cpu/mmap.cpp:91-145 (page-aligned, sequential advice); read-ahead of
+// the next y window.
 struct MappedFile {
     std::string path;
     NpyHeader h;
@@ -223,6 +225,44 @@ struct MappedFile {
         (void)::madvise(b, w->bytes, MADV_SEQUENTIAL);
         return w;
     }
+    // Elements [first, first+count) read into dst by up to `threads` threads,
+    // each with pread over its own piece (page cache to pinned memory without
+    // the page faults of a mapping).
+    void read(uint64_t first, uint64_t count, char *dst, unsigned threads) const {
+        const uint64_t off = h.offset + first * h.item;
+        const size_t bytes = size_t(count * h.item);
+        const size_t min_piece = size_t(8) << 20;
+        const unsigned t =
+            unsigned(std::min<size_t>(threads, std::max<size_t>(1, bytes / min_piece)));
+        std::atomic<int> failed{0};
+        auto piece = [&](size_t a, size_t b) {
+            while (a < b) {
+                const ssize_t got = ::pread(fd, dst + a, b - a, off_t(off + a));
+                if (got < 0 && errno == EINTR)
+                    continue;
+                if (got <= 0) {
+                    failed.store(1);
+                    return;
+                }
+                a += size_t(got);
+            }
+        };
+        if (t <= 1) {
+            piece(0, bytes);
+        } else {
+            std::vector<std::thread> pool;
+            const size_t step = (bytes + t - 1) / t;
+            for (unsigned k = 0; k < t; ++k) {
+                const size_t a = k * step, b = std::min(bytes, a + step);
+                if (a < b)
+                    pool.emplace_back(piece, a, b);
+            }
+            for (auto &th : pool)
+                th.join();
+        }
+        if (failed.load())
+            runtime("NPY read failed: " + path);
+    }
     void prefetch(uint64_t first, uint64_t count) const {
         if (count)
             (void)::posix_fadvise(fd, off_t(h.offset + first * h.item), off_t(count * h.item),
@@ -236,26 +276,6 @@ __global__ void __launch_bounds__(256) narrow_kernel(const double *in, float *ou
     for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n;
          i += uint64_t(gridDim.x) * 256)
         out[i] = __double2float_rn(in[i]);
-}
-
-// Copy `bytes` from src to dst with up to `threads` threads (page cache to
-// pinned memory runs several times faster than one thread's memcpy).
-void parallel_copy(char *dst, const char *src, size_t bytes, unsigned threads) {
-    const size_t min_piece = size_t(8) << 20;
-    const unsigned t = unsigned(std::min<size_t>(threads, std::max<size_t>(1, bytes / min_piece)));
-    if (t <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const size_t piece = (bytes + t - 1) / t;
-    for (unsigned k = 0; k < t; ++k) {
-        const size_t a = k * piece, b = std::min(bytes, a + piece);
-        if (a < b)
-            pool.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
-    }
-    for (auto &th : pool)
-        th.join();
 }
 
 struct PipeSlot {
@@ -503,7 +523,7 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
                 }
             }
         });
-        const unsigned copy_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+        const unsigned copy_threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         bf_config cfg = *cfg_in;
         try {
             yf.prefetch(0, std::min(chunk, n) * m);
@@ -523,11 +543,8 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
                 const double t0 = now_seconds();
                 // Window → pinned, as stored.
                 const size_t ybytes = size_t(count * m * yf.h.item);
-                {
-                    auto w = yf.window(first * m, count * m);
-                    stats.max_mapped_bytes = std::max<uint64_t>(stats.max_mapped_bytes, w->bytes);
-                    parallel_copy(s.pinned, w->data(), ybytes, copy_threads);
-                }
+                yf.read(first * m, count * m, s.pinned, copy_threads);
+                stats.max_mapped_bytes = std::max<uint64_t>(stats.max_mapped_bytes, ybytes);
                 const size_t pbytes = pf ? size_t(count * P * pf->h.item) : 0;
                 const size_t p_pinned = align256(size_t(in_bytes));
                 if (pf) {

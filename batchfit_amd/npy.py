@@ -2,7 +2,7 @@
 
 ``fit_npy(x_path, y_path, model, p0_path=None, bounds=None, ...)`` fits every
 row of an NPY file y[N,M] (float32 or float64) against x[M] without loading
-the file: chunks are memory-mapped, staged through a ring of pinned slots,
+the file: chunks are read by several threads into a ring of pinned slots,
 fitted on the device (float64 is narrowed there) and returned in order — the
 pipeline of the reference's ``metal::fit_npy`` (src/metal/pipeline.cpp:70-265)
 with the guess and the conversion moved to the device. Without a ``consumer``

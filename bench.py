@@ -479,8 +479,9 @@ def bench_serving(S, args):
 
 def bench_cube(S, args):
     """512×512×1024 pseudo-Voigt cube: generated in HBM, written to an NPY
-    file, then fitted from the file by the streaming pipeline (memory-mapped
-    chunks → pinned ring → HBM → device guess and fit → host, in order)."""
+    file, then fitted from the file by the streaming pipeline (chunks read by
+    parallel pread → pinned ring → HBM → device guess and fit → host, in
+    order)."""
     from batchfit_amd.npy import fit_npy
     from batchfit_amd.ragged import DeviceArray
     n, m = 512 * 512, 1024
@@ -515,7 +516,7 @@ def bench_cube(S, args):
             "max_in_flight": stats["max_in_flight"],
             "prepare_seconds": stats["prepare_seconds"],
             "status_counts": np.bincount(result.status, minlength=6).tolist(),
-            "path": "NPY file (page cache) -> memory-mapped windows -> pinned ring -> HBM -> "
+            "path": "NPY file (page cache) -> parallel pread -> pinned ring -> HBM -> "
                     "device guess + fit -> host, in order (bf_hip_fit_npy)"}
 
 
