@@ -140,7 +140,7 @@ def _make_config(n, m, lower, upper, max_iterations, numerics):
 
 _MAPPINGS = {"auto": (0, 0, 0), "lane": (1, 0, 0), "group8": (2, 8, 0), "group16": (2, 16, 0),
              "group32": (2, 32, 0), "group64": (2, 64, 0), "wave": (2, 64, 0),
-             "group8_streamed": (2, 8, 1), "group16_streamed": (2, 16, 1),
+             "group4_streamed": (2, 4, 1), "group8_streamed": (2, 8, 1), "group16_streamed": (2, 16, 1),
              "group32_streamed": (2, 32, 1), "group64_streamed": (2, 64, 1)}
 
 
@@ -178,7 +178,7 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
     partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
     ``mapping`` ("auto", "lane", "group8", "group16", "group32", "group64"; a "_streamed"
     suffix reads the points from memory on every pass instead of registers,
-    with the same results) and
+    with the same results; "group4_streamed" streams 4-lane groups) and
     ``max_iterations`` (default 100, cpu/solver.hpp:27).
     """
     if model not in _PARAMETERS:

@@ -8,11 +8,14 @@ workload), started from the device guess as in production (so the spread
 of iteration counts between the fits that share a wave is measured too), and
 solved by the LM kernel with each candidate mapping — 8, 16, 32 or 64 lanes
 per fit, with the points either in a register tile (the smallest power of two
-≥ M/G, up to 32 per lane) or read from memory on every pass — in the
+≥ M/G, up to 32 per lane) or read from memory on every pass, or 4 lanes per
+fit streamed — in the
 production numerics (compensated partials, FP64 solve). A fit's result
 depends on G and M only, so the table changes speed, not results. The LM
 kernel time is read from the library's events; the fastest mapping wins
-each (model, M). Adjacent winners are merged into ranges and,
+each (model, M), and a dynamic programme over the ladder trades a slower
+mapping against a switch between mappings (a switch costs 10 %). The
+resulting ranges are,
 with --write, emitted as batchfit_amd/csrc/bf_tuned.hpp (the table both the
 uniform and the ragged paths use), followed by a rebuild. The measured table
 is printed as JSON.
@@ -37,8 +40,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MODELS = ("gaussian", "lorentzian", "pseudo_voigt", "two_lorentzian")
 LADDER = (32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096)
 # (lanes per fit, streamed): points in a register tile, or read from memory
-# on every pass.
-CANDIDATES = ((8, 0), (16, 0), (32, 0), (64, 0), (8, 1), (16, 1), (32, 1), (64, 1))
+# on every pass (4-lane groups always stream).
+CANDIDATES = ((8, 0), (16, 0), (32, 0), (64, 0), (4, 1), (8, 1), (16, 1), (32, 1), (64, 1))
 RANGES = {  # parameters drawn per fit (the ragged benchmark workload's ranges)
     "gaussian": ([0.5, -3, 0.085], [5, 3, 0.85]),
     "lorentzian": ([0.5, -3, 0.1], [5, 3, 1.0]),
@@ -180,7 +183,7 @@ def write_header(rng: dict, path: str) -> None:
     lines = [
         "// bf_tuned.hpp — generated by `python -m batchfit_amd.autotune --write`: lanes per",
         "// fit by (model, M), measured on gfx950 (LM kernel time of the production numerics",
-        "// from true parameters; see batchfit_amd/autotune.py). Rows: gaussian, lorentzian,",
+        "// from device-guess starts; see batchfit_amd/autotune.py). Rows: gaussian, lorentzian,",
         "// pseudo_voigt, two_lorentzian; each entry {largest M, lanes, streamed}, the last",
         "// one open.",
         "#pragma once",

@@ -160,7 +160,7 @@ template <> struct ModelConsts<TWO_LORENTZIAN> {
 };
 
 // ---------------------------------------------------------------------------
-// Cross-lane all-reduce within aligned groups of G lanes (G = 8, 16, 32, 64).
+// Cross-lane all-reduce within aligned groups of G lanes (G = 4, 8, 16, 32, 64).
 // Steps: quad_perm xor1, quad_perm xor2, row_half_mirror (8), row_mirror (16),
 // ds_swizzle xor16 (32), permlane32_swap (64). Mirror partners of a lane lie
 // in the other half of its 8/16-lane block, so after each step every lane
@@ -322,7 +322,8 @@ template <int STEP> __device__ __forceinline__ double xlane_d(double v) {
 template <int G> __device__ __forceinline__ float group_sum_f(float v) {
     v += xlane_f<1>(v);
     v += xlane_f<2>(v);
-    v += xlane_f<4>(v);
+    if constexpr (G >= 8)
+        v += xlane_f<4>(v);
     if constexpr (G >= 16)
         v += xlane_f<8>(v);
     if constexpr (G >= 32)
@@ -335,7 +336,8 @@ template <int G> __device__ __forceinline__ float group_sum_f(float v) {
 template <int G> __device__ __forceinline__ double group_sum_d(double v) {
     v += xlane_d<1>(v);
     v += xlane_d<2>(v);
-    v += xlane_d<4>(v);
+    if constexpr (G >= 8)
+        v += xlane_d<4>(v);
     if constexpr (G >= 16)
         v += xlane_d<8>(v);
     if constexpr (G >= 32)
@@ -348,7 +350,8 @@ template <int G> __device__ __forceinline__ double group_sum_d(double v) {
 template <int G> __device__ __forceinline__ int group_or(int v) {
     v |= xlane<1>(v);
     v |= xlane<2>(v);
-    v |= xlane<4>(v);
+    if constexpr (G >= 8)
+        v |= xlane<4>(v);
     if constexpr (G >= 16)
         v |= xlane<8>(v);
     if constexpr (G >= 32)
@@ -362,7 +365,8 @@ template <int G> __device__ __forceinline__ int group_or(int v) {
 template <int G> __device__ __forceinline__ float group_max_f(float v) {
     v = fmaxf(v, xlane_f<1>(v));
     v = fmaxf(v, xlane_f<2>(v));
-    v = fmaxf(v, xlane_f<4>(v));
+    if constexpr (G >= 8)
+        v = fmaxf(v, xlane_f<4>(v));
     if constexpr (G >= 16)
         v = fmaxf(v, xlane_f<8>(v));
     if constexpr (G >= 32)
@@ -374,7 +378,8 @@ template <int G> __device__ __forceinline__ float group_max_f(float v) {
 template <int G> __device__ __forceinline__ int group_max_i(int v) {
     v = max(v, xlane<1>(v));
     v = max(v, xlane<2>(v));
-    v = max(v, xlane<4>(v));
+    if constexpr (G >= 8)
+        v = max(v, xlane<4>(v));
     if constexpr (G >= 16)
         v = max(v, xlane<8>(v));
     if constexpr (G >= 32)
@@ -386,7 +391,8 @@ template <int G> __device__ __forceinline__ int group_max_i(int v) {
 template <int G> __device__ __forceinline__ int group_min_i(int v) {
     v = min(v, xlane<1>(v));
     v = min(v, xlane<2>(v));
-    v = min(v, xlane<4>(v));
+    if constexpr (G >= 8)
+        v = min(v, xlane<4>(v));
     if constexpr (G >= 16)
         v = min(v, xlane<8>(v));
     if constexpr (G >= 32)
@@ -398,7 +404,8 @@ template <int G> __device__ __forceinline__ int group_min_i(int v) {
 template <int G> __device__ __forceinline__ float group_min_f(float v) {
     v = fminf(v, xlane_f<1>(v));
     v = fminf(v, xlane_f<2>(v));
-    v = fminf(v, xlane_f<4>(v));
+    if constexpr (G >= 8)
+        v = fminf(v, xlane_f<4>(v));
     if constexpr (G >= 16)
         v = fminf(v, xlane_f<8>(v));
     if constexpr (G >= 32)

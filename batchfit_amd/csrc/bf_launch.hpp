@@ -120,14 +120,17 @@ inline TunedRange tuned_mapping(int model, uint32_t m) {
 inline int tuned_group(int model, uint32_t m) { return tuned_mapping(model, m).lanes; }
 
 // Instantiated plans: lane-per-fit with every numerics form; 64-lane groups
-// with tiles 0 (stream), 2 … 32 and every numerics form; 16- and 32-lane
-// groups with tiles 2 … 32 in the plain FP32 and compensated-mixed forms.
+// with tiles 0 (stream), 2 … 32 and every numerics form; 8-, 16- and 32-lane
+// groups streamed or with tiles 2 … 32, and 4-lane groups streamed, in the
+// plain FP32 and compensated-mixed forms.
 constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
     const bool tile = R == 2 || R == 4 || R == 8 || R == 16 || R == 32;
     if (G == 1)
         return R == 0;
     if (G == 64)
         return R == 0 || tile;
+    if (G == 4)
+        return R == 0 && comp == mixed;
     if (G == 8 || G == 16 || G == 32)
         return (R == 0 || tile) && comp == mixed;
     return false;

@@ -7,6 +7,7 @@ namespace bfhip {
 #define BF_DECLARE_GROUPS(M)                                                                   \
     template <> hipError_t launch_lm_group<M, 1>(const Plan &, const FitArgs &, hipStream_t);  \
     template <> hipError_t launch_lm_group<M, 8>(const Plan &, const FitArgs &, hipStream_t);  \
+    template <> hipError_t launch_lm_group<M, 4>(const Plan &, const FitArgs &, hipStream_t);  \
     template <> hipError_t launch_lm_group<M, 16>(const Plan &, const FitArgs &, hipStream_t); \
     template <> hipError_t launch_lm_group<M, 32>(const Plan &, const FitArgs &, hipStream_t); \
     template <> hipError_t launch_lm_group<M, 64>(const Plan &, const FitArgs &, hipStream_t);
@@ -22,6 +23,8 @@ BF_DECLARE_GROUPS(3)
         switch (plan.G) {                                                                      \
         case 1:                                                                                \
             return launch_lm_group<M, 1>(plan, args, stream);                                  \
+        case 4:                                                                                \
+            return launch_lm_group<M, 4>(plan, args, stream);                                  \
         case 8:                                                                                \
             return launch_lm_group<M, 8>(plan, args, stream);                                  \
         case 16:                                                                               \

@@ -267,14 +267,16 @@ struct RaggedClass {
     int G, R;
     uint32_t max_points;
 };
-constexpr int kRaggedClassCount = 12;
+constexpr int kRaggedClassCount = 13;
 constexpr RaggedClass kRaggedClasses[kRaggedClassCount] = {
     {16, 16, 256},  {16, 32, 512},        {32, 16, 512},        {32, 32, 1024},
     {64, 16, 1024}, {64, 32, 2048},       {64, 0, 0xffffffffu}, {16, 0, 0xffffffffu},
-    {32, 0, 0xffffffffu}, {8, 16, 128},   {8, 32, 256},         {8, 0, 0xffffffffu}};
-// Enough queue counters for every (model, class) run.
-constexpr int kRaggedCounters = 48;
-static_assert(4 * kRaggedClassCount <= kRaggedCounters, "one counter per run");
+    {32, 0, 0xffffffffu}, {8, 16, 128},   {8, 32, 256},         {8, 0, 0xffffffffu},
+    {4, 0, 0xffffffffu}};
+// Enough queue counters for every run: one per (model, class) bucket, plus
+// at most 8 more from splitting buckets heavier than an eighth of the batch.
+constexpr int kRaggedCounters = 64;
+static_assert(4 * kRaggedClassCount + 8 <= kRaggedCounters, "one counter per run");
 inline int ragged_class_of(uint32_t model, uint32_t m) {
     const TunedRange t = tuned_mapping(int(model), m);
     // Beyond 32 points per lane the points stream, as in a uniform launch.
@@ -307,7 +309,8 @@ struct RaggedRun {
 };
 
 // Host view of a ragged batch: validated descriptors bucketed by (model,
-// class), each bucket longest-first.
+// class), each bucket longest-first and cut into runs of at most about an
+// eighth of the batch's work.
 struct RaggedLayout {
     std::vector<uint32_t> order;
     std::vector<RaggedRun> runs;

@@ -56,6 +56,7 @@ hipError_t ragged_by_class(int cls, const RaggedArgs &args, uint32_t blocks, hip
     case 9: return ragged_go<MODEL, 9, MIXED>(args, blocks, stream);
     case 10: return ragged_go<MODEL, 10, MIXED>(args, blocks, stream);
     case 11: return ragged_go<MODEL, 11, MIXED>(args, blocks, stream);
+    case 12: return ragged_go<MODEL, 12, MIXED>(args, blocks, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -103,7 +104,8 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
         {&ragged_symbol<BF_MODEL, 8, false>, &ragged_symbol<BF_MODEL, 8, true>},
         {&ragged_symbol<BF_MODEL, 9, false>, &ragged_symbol<BF_MODEL, 9, true>},
         {&ragged_symbol<BF_MODEL, 10, false>, &ragged_symbol<BF_MODEL, 10, true>},
-        {&ragged_symbol<BF_MODEL, 11, false>, &ragged_symbol<BF_MODEL, 11, true>}};
+        {&ragged_symbol<BF_MODEL, 11, false>, &ragged_symbol<BF_MODEL, 11, true>},
+        {&ragged_symbol<BF_MODEL, 12, false>, &ragged_symbol<BF_MODEL, 12, true>}};
     if (cls < 0 || cls >= kRaggedClassCount)
         return hipErrorInvalidValue;
     const void *sym = table[cls][mixed ? 1 : 0]();
@@ -122,7 +124,8 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
             {&pool_symbol<BF_MODEL, 8, false>, &pool_symbol<BF_MODEL, 8, true>},
             {&pool_symbol<BF_MODEL, 9, false>, &pool_symbol<BF_MODEL, 9, true>},
             {&pool_symbol<BF_MODEL, 10, false>, &pool_symbol<BF_MODEL, 10, true>},
-            {&pool_symbol<BF_MODEL, 11, false>, &pool_symbol<BF_MODEL, 11, true>}};
+            {&pool_symbol<BF_MODEL, 11, false>, &pool_symbol<BF_MODEL, 11, true>},
+        {&pool_symbol<BF_MODEL, 12, false>, &pool_symbol<BF_MODEL, 12, true>}};
         e = hipFuncGetAttributes(&attr, pools[cls][mixed ? 1 : 0]());
         static constexpr const void *(*guessed[kRaggedClassCount][2])() = {
             {&pool_symbol_guessed<BF_MODEL, 0, false>, &pool_symbol_guessed<BF_MODEL, 0, true>},
@@ -136,7 +139,8 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
             {&pool_symbol_guessed<BF_MODEL, 8, false>, &pool_symbol_guessed<BF_MODEL, 8, true>},
             {&pool_symbol_guessed<BF_MODEL, 9, false>, &pool_symbol_guessed<BF_MODEL, 9, true>},
             {&pool_symbol_guessed<BF_MODEL, 10, false>, &pool_symbol_guessed<BF_MODEL, 10, true>},
-            {&pool_symbol_guessed<BF_MODEL, 11, false>, &pool_symbol_guessed<BF_MODEL, 11, true>}};
+            {&pool_symbol_guessed<BF_MODEL, 11, false>, &pool_symbol_guessed<BF_MODEL, 11, true>},
+        {&pool_symbol_guessed<BF_MODEL, 12, false>, &pool_symbol_guessed<BF_MODEL, 12, true>}};
         if (e == hipSuccess)
             e = hipFuncGetAttributes(&attr, guessed[cls][mixed ? 1 : 0]());
         static constexpr const void *(*covs[kRaggedClassCount][2])() = {
@@ -151,7 +155,8 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
             {&cov_symbol<BF_MODEL, 8, false>, &cov_symbol<BF_MODEL, 8, true>},
             {&cov_symbol<BF_MODEL, 9, false>, &cov_symbol<BF_MODEL, 9, true>},
             {&cov_symbol<BF_MODEL, 10, false>, &cov_symbol<BF_MODEL, 10, true>},
-            {&cov_symbol<BF_MODEL, 11, false>, &cov_symbol<BF_MODEL, 11, true>}};
+            {&cov_symbol<BF_MODEL, 11, false>, &cov_symbol<BF_MODEL, 11, true>},
+        {&cov_symbol<BF_MODEL, 12, false>, &cov_symbol<BF_MODEL, 12, true>}};
         if (e == hipSuccess)
             e = hipFuncGetAttributes(&attr, covs[cls][mixed ? 1 : 0]());
     }

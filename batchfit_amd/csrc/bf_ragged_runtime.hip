@@ -106,19 +106,29 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         bucket[d.model * kRaggedClassCount + ragged_class_of(d.model, d.points)].push_back(i);
     }
     out.order.reserve(fits);
-    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
-        auto &v = bucket[b];
-        if (v.empty())
-            continue;
-        sort_longest_first(v, fit_table);
-        RaggedRun r{uint32_t(b / kRaggedClassCount), b % kRaggedClassCount,
-                    uint32_t(out.order.size()), uint32_t(v.size()), 0, 0, 0.0, {}};
-        for (uint32_t i : v) {
-            r.guesses += fit_table[i].p0_row == 0xffffffffu;
-            r.max_points = std::max(r.max_points, fit_table[i].points);
+    // Relative work: points × parameters-weighted passes (a P=8 fit costs
+    // about three times a P=3 one per point).
+    static constexpr double kWeight[4] = {1.0, 1.0, 1.5, 3.5};
+    double total = 0.0;
+    for (uint32_t i = 0; i < fits; ++i)
+        total += double(fit_table[i].points) * kWeight[fit_table[i].model];
+    // A bucket heavier than an eighth of the batch becomes several runs over
+    // consecutive slices of its longest-first order, so that the branches of
+    // the graph can balance (a fit's result does not depend on its run).
+    // Pieces hold at least 2^26 weighted points (about 0.7 ms of work on
+    // MI355X), so small batches are not cut into launches that cost more
+    // than the balance gains.
+    const double piece_cost = std::max(total / 8.0, 67108864.0);
+    auto add_run = [&](uint32_t model, int cls, const uint32_t *v, size_t n) {
+        RaggedRun r{model, cls, uint32_t(out.order.size()), uint32_t(n), 0, 0, 0.0, {}};
+        double points = 0.0;
+        for (size_t k = 0; k < n; ++k) {
+            r.guesses += fit_table[v[k]].p0_row == 0xffffffffu;
+            r.max_points = std::max(r.max_points, fit_table[v[k]].points);
+            points += fit_table[v[k]].points;
         }
         // v is longest first, so each octave of M is a contiguous slice.
-        for (size_t a = 0; a < v.size();) {
+        for (size_t a = 0; a < n;) {
             const uint32_t top = fit_table[v[a]].points;
             uint32_t below = 0;  // the octave is (below, 2·below]
             if (top > 256)
@@ -126,22 +136,37 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
                 }
             size_t end = a;
             uint32_t guessed = 0;
-            for (; end < v.size() && fit_table[v[end]].points > below; ++end)
+            for (; end < n && fit_table[v[end]].points > below; ++end)
                 guessed += fit_table[v[end]].p0_row == 0xffffffffu;
             if (guessed)
                 r.guess_slices.push_back({uint32_t(a), uint32_t(end - a), top});
             a = end;
         }
-        // Relative work: points × parameters-weighted passes (a P=8 fit costs
-        // about three times a P=3 one per point).
-        static constexpr double kWeight[4] = {1.0, 1.0, 1.5, 3.5};
-        double points = 0.0;
-        for (uint32_t i : v)
-            points += fit_table[i].points;
-        r.cost = points * kWeight[r.model];
-        out.order.insert(out.order.end(), v.begin(), v.end());
-        out.runs.push_back(r);
+        r.cost = points * kWeight[model];
+        out.order.insert(out.order.end(), v, v + n);
+        out.runs.push_back(std::move(r));
+    };
+    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
+        auto &v = bucket[b];
+        if (v.empty())
+            continue;
+        sort_longest_first(v, fit_table);
+        const uint32_t model = uint32_t(b / kRaggedClassCount);
+        const int cls = b % kRaggedClassCount;
+        size_t a = 0;
+        double acc = 0.0;
+        for (size_t k = 0; k < v.size(); ++k) {
+            acc += double(fit_table[v[k]].points) * kWeight[model];
+            if (acc >= piece_cost && k + 1 < v.size()) {
+                add_run(model, cls, v.data() + a, k + 1 - a);
+                a = k + 1;
+                acc = 0.0;
+            }
+        }
+        add_run(model, cls, v.data() + a, v.size() - a);
     }
+    if (out.runs.size() > size_t(kRaggedCounters))
+        return "internal: more runs than queue counters";
     out.cost_order.resize(out.runs.size());
     std::iota(out.cost_order.begin(), out.cost_order.end(), size_t(0));
     std::stable_sort(out.cost_order.begin(), out.cost_order.end(),

@@ -99,13 +99,14 @@ int choose_plan(uint32_t model, uint32_t m, uint32_t numerics, const bf_launch *
         streamed = launch->streamed != 0;
     else if (mapping != BF_MAP_GROUP && mapping != BF_MAP_AUTO)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown mapping");
-    if (mapping == BF_MAP_AUTO && g != 64 && !plan_instantiated(g, 2, plan.comp, plan.mixed))
-        g = 64;  // 8/16/32-lane groups exist for numerics 0 and 5 only
-    if (g != 8 && g != 16 && g != 32 && g != 64)
-        return fail(err, errlen, BF_INVALID_ARGUMENT, "group_lanes must be 8, 16, 32 or 64");
+    if (mapping == BF_MAP_AUTO && g != 64 &&
+        !plan_instantiated(g, g == 4 ? 0 : 2, plan.comp, plan.mixed))
+        g = 64;  // 4/8/16/32-lane groups exist for numerics 0 and 5 only
+    if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "group_lanes must be 4, 8, 16, 32 or 64");
     const uint32_t per_lane = (m + uint32_t(g) - 1) / uint32_t(g);
     int r = pow2_ceil(per_lane < 2 ? 2 : per_lane);
-    if (r > kMaxTile || streamed)
+    if (r > kMaxTile || streamed || g == 4)  // 4-lane groups always stream
         r = 0;
     plan.G = g;
     plan.R = r;

@@ -73,7 +73,7 @@ enum bf_mapping {
 
 typedef struct bf_launch {
     uint32_t mapping;     /* bf_mapping */
-    uint32_t group_lanes; /* 8, 16, 32 or 64 for BF_MAP_GROUP; 0 = tuned */
+    uint32_t group_lanes; /* 4, 8, 16, 32 or 64 for BF_MAP_GROUP (4 always streams); 0 = tuned */
     uint32_t point_major; /* BF_MAP_LANE only: 1 = transpose y on the device first */
     uint32_t streamed;    /* BF_MAP_GROUP only: 1 = read the points from memory on every
                              pass instead of holding them in registers (same results) */

@@ -18,15 +18,16 @@ import datasets
 
 pytestmark = pytest.mark.gpu
 
-MAPPINGS = ["lane", "group8", "group16", "group32", "group64"]
+MAPPINGS = ["lane", "group4_streamed", "group8", "group16", "group32", "group64"]
 NUMERICS = [0, 5]  # FP32 as the reference GPU kernels; compensated FP32 + FP64 solve
 P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
 
 def mappings_for(m):
-    """Mappings whose register tile (at most 32 points per lane) holds M."""
+    """Mappings whose register tile (at most 32 points per lane) holds M, and
+    the 4-lane groups, which always stream."""
     return [mp for mp in MAPPINGS
-            if mp in ("lane", "group64") or (mp == "group32" and m <= 1024)
+            if mp in ("lane", "group4_streamed", "group64") or (mp == "group32" and m <= 1024)
             or (mp == "group16" and m <= 512) or (mp == "group8" and m <= 256)]
 
 
@@ -343,7 +344,8 @@ def test_pathology_matrix(hip, golden, model):
     y = g[f"path_{model}_y"]
     for mapping in MAPPINGS:
         for numerics in (0, 1, 2, 3, 5, 7):
-            if mapping in ("group8", "group16", "group32") and numerics not in (0, 5):
+            if mapping in ("group4_streamed", "group8", "group16", "group32") and \
+                    numerics not in (0, 5):
                 continue
             r = hip.fit(x, y, model, inits, None, "hip", mapping=mapping, numerics=numerics)
             assert list(r.status) == [4, 4, 3, 3, 3, 0, 5, 0, 3], (mapping, numerics, r.status)
