@@ -309,8 +309,8 @@ struct RaggedRun {
 };
 
 // Host view of a ragged batch: validated descriptors bucketed by (model,
-// class), each bucket longest-first and cut into runs of at most about an
-// eighth of the batch's work.
+// class), each bucket longest-first and, when asked, cut into runs of at
+// most about an eighth of the batch's work.
 struct RaggedLayout {
     std::vector<uint32_t> order;
     std::vector<RaggedRun> runs;
@@ -332,8 +332,12 @@ struct RaggedDevice {
 };
 
 // Validate and bucket; returns an empty string or the reason for rejection.
+// `split`: cut buckets heavier than an eighth of the batch into several runs
+// (for the branches of a large batch's graph; the server's batches stay
+// whole, which kept their tail latency lower).
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
-                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out);
+                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out,
+                          bool split);
 
 // Device inputs and outputs of one ragged batch (rows of 8 for params and
 // stderr, 64 for cov; cov/stderr/ustatus null without uncertainties).

@@ -6,6 +6,7 @@
 #include "bf_ragged.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <cstdlib>
 #include <numeric>
 #include <vector>
@@ -84,7 +85,8 @@ void sort_longest_first(std::vector<uint32_t> &v, const bf_fit_desc *t) {
 }  // namespace
 
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
-                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out) {
+                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out,
+                          bool split) {
     out = RaggedLayout{};
     if (fits && (!fit_table || !grid_table || !grids || !configs))
         return "descriptor tables are required";
@@ -118,7 +120,8 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
     // Pieces hold at least 2^26 weighted points (about 0.7 ms of work on
     // MI355X), so small batches are not cut into launches that cost more
     // than the balance gains.
-    const double piece_cost = std::max(total / 8.0, 67108864.0);
+    const double piece_cost =
+        split ? std::max(total / 8.0, 67108864.0) : std::numeric_limits<double>::infinity();
     auto add_run = [&](uint32_t model, int cls, const uint32_t *v, size_t n) {
         RaggedRun r{model, cls, uint32_t(out.order.size()), uint32_t(n), 0, 0, 0.0, {}};
         double points = 0.0;
@@ -389,7 +392,8 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if (fits && !config_table)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "descriptor tables are required");
     auto p = new bf_ragged_plan_s();
-    const std::string why = ragged_layout(fits, fit_table, grids, grid_table, configs, p->layout);
+    const std::string why =
+        ragged_layout(fits, fit_table, grids, grid_table, configs, p->layout, true);
     if (!why.empty()) {
         delete p;
         return fail(err, errlen, BF_INVALID_ARGUMENT, why);

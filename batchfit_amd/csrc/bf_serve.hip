@@ -254,7 +254,7 @@ void Server::launch(Slot &slot) {
                                   uint32_t(r.out_row + k)};
     }
     RaggedLayout layout;
-    const std::string why = ragged_layout(uint32_t(rows), fits, nreq, grids, nreq, layout);
+    const std::string why = ragged_layout(uint32_t(rows), fits, nreq, grids, nreq, layout, false);
     if (!why.empty()) {
         slot.rc = BF_INVALID_ARGUMENT;
         slot.message = why;
