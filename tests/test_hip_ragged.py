@@ -83,6 +83,24 @@ def test_ragged_equals_uniform_large_streamed_class(hip, model, m):
         assert same(u, v), (model, m, name)
 
 
+def test_ragged_split_runs_equal_uniform(hip):
+    """A class heavier than an eighth of the batch and than 2^26 weighted
+    points (40,000 two-peak fits of M = 2048: 2.9e8) is cut into several runs
+    over consecutive slices of its order; every fit keeps the bytes of the
+    uniform launch, and the small batch beside it too."""
+    from batchfit_amd.ragged import fit_ragged
+    x, y, _, bounds, _ = datasets.comparison_set("two_lorentzian", 40000, 2048)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    xs, ys, _, bs, _ = datasets.comparison_set("gaussian", 50, 129)
+    xs, ys = xs.astype(np.float32), ys.astype(np.float32)
+    big, small = fit_ragged([(x, y, "two_lorentzian", None, bounds),
+                             (xs, ys, "gaussian", None, bs)])
+    for u, v in zip(hip.fit(x, y, "two_lorentzian", None, bounds, "hip"), big):
+        assert same(u, v)
+    for u, v in zip(hip.fit(xs, ys, "gaussian", None, bs, "hip"), small):
+        assert same(u, v)
+
+
 def test_ragged_uncertainties_and_failures(hip, oracle_port):
     from batchfit_amd.ragged import fit_ragged
     x, y, p0, bounds, _ = datasets.comparison_set("gaussian", 20, 129)
