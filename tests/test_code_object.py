@@ -1,0 +1,70 @@
+"""Kernel resources of the built library, read from its gfx950 code objects
+(no device needed): the device guess kernels keep everything in registers
+and LDS (no private segment: no stack frames, no out-of-line calls whose
+arguments live in scratch), and the production LM kernel of the metric
+workload (pseudo-Voigt, 8 lanes, streamed, compensated, FP64 solve, guessed
+starts) neither spills nor uses scratch."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+LLVM = "/opt/rocm/llvm/bin"
+LIB = os.path.join(os.path.dirname(__file__), "..", "batchfit_amd", "libbatchfit_hip.so")
+MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+TARGET = "hipv4-amdgcn-amd-amdhsa--gfx950"
+
+
+def kernel_resources(tmp_path):
+    """{kernel symbol: {"private": bytes, "spill": VGPRs}} over every bundle of
+    the library's .hip_fatbin section (one bundle per translation unit)."""
+    fat = tmp_path / "fat.bin"
+    subprocess.run([f"{LLVM}/llvm-objcopy", f"--dump-section=.hip_fatbin={fat}", LIB,
+                    str(tmp_path / "rest.so")], check=True)
+    data = fat.read_bytes()
+    starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
+    kernels = {}
+    for i, a in enumerate(starts):
+        b = starts[i + 1] if i + 1 < len(starts) else len(data)
+        part, co = tmp_path / f"b{i}.bin", tmp_path / f"b{i}.co"
+        part.write_bytes(data[a:b])
+        subprocess.run([f"{LLVM}/clang-offload-bundler", "--unbundle", "--type=o",
+                        f"--targets={TARGET}", f"--input={part}", f"--output={co}"], check=True)
+        notes = subprocess.run([f"{LLVM}/llvm-readelf", "--notes", str(co)], check=True,
+                               capture_output=True, text=True).stdout
+        name = None
+        for line in notes.splitlines():
+            line = line.strip()
+            if line.startswith(".name:"):
+                name = line.split(":", 1)[1].strip()
+                kernels[name] = {}
+            elif name and line.startswith(".private_segment_fixed_size:"):
+                kernels[name]["private"] = int(line.split(":")[1])
+            elif name and line.startswith(".vgpr_spill_count:"):
+                kernels[name]["spill"] = int(line.split(":")[1])
+    return kernels
+
+
+@pytest.fixture(scope="module")
+def resources(tmp_path_factory):
+    if not os.path.exists(LIB):
+        pytest.skip("library not built")
+    for tool in ("llvm-objcopy", "clang-offload-bundler", "llvm-readelf"):
+        if not shutil.which(f"{LLVM}/{tool}"):
+            pytest.skip(f"{tool} not found")
+    return kernel_resources(tmp_path_factory.mktemp("code_object"))
+
+
+def test_guess_kernels_use_no_private_memory(resources):
+    guess = {k: v for k, v in resources.items() if "guess_kernel" in k or "guess_ragged_kernel" in k}
+    assert len(guess) == 8, sorted(guess)  # uniform and ragged, four models
+    for name, r in guess.items():
+        assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)
+
+
+def test_metric_lm_kernel_has_no_spills(resources):
+    name = "_ZN5bfhip14lm_pool_kernelILi2ELi8ELb1ELb1ELb0EEEvNS_7FitArgsE"
+    assert name in resources
+    assert resources[name].get("private", 0) == 0 and resources[name].get("spill", 0) == 0
