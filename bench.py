@@ -176,7 +176,8 @@ def bench_metric(S, args):
             "bound": "mfma", "engine": "VALU FP32 (MFMA unused: P<=8 normal equations)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(),
-            "kernel": f"lm_kernel<pseudo_voigt, {form}, compensated, FP64 solve>",
+            "kernel": (f"{'lm_pool_kernel' if tile == 0 and lanes > 1 else 'lm_kernel'}"
+                       f"<pseudo_voigt, {form}, compensated, FP64 solve>"),
             "algorithmic_flop_per_launch": flops_lm,
             "algorithmic_bytes_per_launch": n * (4 * m + 16 + 12),
             "kernel_ms_hip_events": lm_ms,
