@@ -436,6 +436,26 @@ __device__ bool prominent_wave(const float *s, int m, int j, float h, float t, i
     return false;
 }
 
+// One side of the prominence test of prominent_wave for the candidate at j
+// of height h, walked by its own lane over at most kLocalWalk points in
+// direction dir (±1): 1 if a point with h − v > t comes first (the side
+// passes), 2 if a point above h or the edge comes first (it fails), 0 if
+// neither lies within reach.
+constexpr int kLocalWalk = 6;
+__device__ __forceinline__ int local_side(const float *s, int m, int j, float h, float t, int dir) {
+    for (int d = 1; d <= kLocalWalk; ++d) {
+        const int k = j + dir * d;
+        if (k < 0 || k >= m)
+            return 2;
+        const float v = smooth_at(s, m, k);
+        if (v > h)
+            return 2;
+        if (h - v > t)
+            return 1;
+    }
+    return 0;
+}
+
 // The guess for fit `fit` of `a`, run by one whole wave.
 // Result of one fit's guess, identical in every lane of the wave.
 struct GuessResult {
@@ -595,13 +615,25 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
                     cand = !(bins < 3 || fabsf(x[idx] - primary.center) < separation);
                 }
             }
-            // This step's candidates in increasing index, each tested by the
-            // whole wave; one that is not above the best so far cannot win.
-            for (uint64_t mask = __ballot(cand); mask; mask &= mask - 1) {
+            // Each candidate first walks the nearest kLocalWalk points of both
+            // sides on its own lane (most are settled there, by a higher
+            // point close by); the rest are walked by the whole wave.
+            int state = 0;  // 1: prominent, 2: not, 0: not settled
+            if (cand) {
+                const int ls = local_side(s, m, j, v, t, -1);
+                const int rs = ls == 2 ? 2 : local_side(s, m, j, v, t, 1);
+                state = (ls == 2 || rs == 2) ? 2 : (ls == 1 && rs == 1) ? 1 : 0;
+            }
+            const uint64_t settled_yes = __ballot(cand && state == 1);
+            const uint64_t settled_no = __ballot(cand && state == 2);
+            // This step's candidates in increasing index; one that is not
+            // above the best so far cannot win.
+            for (uint64_t mask = __ballot(cand) & ~settled_no; mask; mask &= mask - 1) {
                 const int l = __builtin_ctzll(mask);
                 const float h = __builtin_bit_cast(
                     float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-                if (h > v_best && prominent_wave(s, m, base + l, h, t, lane)) {
+                if (h > v_best &&
+                    (((settled_yes >> l) & 1) || prominent_wave(s, m, base + l, h, t, lane))) {
                     v_best = h;
                     j_best = base + l;
                 }
