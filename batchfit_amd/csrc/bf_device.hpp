@@ -1,6 +1,6 @@
 // bf_device.hpp — CDNA4 device helpers for the batched LM fitter: finiteness
 // tests on bit patterns, the four peak models with fused value + analytic
-// Jacobian in FP32, and cross-lane sums for groups of 8/16/32/64 lanes.
+// Jacobian in FP32, and cross-lane sums for groups of 4/8/16/32/64 lanes.
 #pragma once
 
 #include <hip/hip_runtime.h>

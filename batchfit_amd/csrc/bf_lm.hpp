@@ -1,8 +1,8 @@
 // bf_lm.hpp — bounded Levenberg–Marquardt kernels for batches of independent
 // spectra on CDNA4 (gfx950).
 //
-// Mapping: a group of G lanes owns one fit (G = 1 lane-per-fit, or 8/16/32/64
-// lanes). With R > 0 the group holds its fit's points in registers, lane l of
+// Mapping: a group of G lanes owns one fit (G = 1 lane-per-fit, or 4/8/16/32/64
+// lanes; 4-lane groups only stream). With R > 0 the group holds its fit's points in registers, lane l of
 // the group owning points l, l+G, …, l+G(R-1); y is read from HBM once per fit.
 // With R = 0 every pass streams x and y from memory (lane-per-fit form).
 //
