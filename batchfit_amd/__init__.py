@@ -8,7 +8,8 @@ device ``"hip"``, which runs hand-written CDNA4 kernels through the C ABI of
 GPU device, ``"hip"`` computes in float32 and returns float32 arrays
 (python/README.md:56-59). The reference's CPU device is not part of this
 package: ``device="cpu"`` is rejected with a message rather than served by a
-fallback.
+fallback, and so is ``device="auto"`` on float64 or integer inputs, which the
+reference's "auto" keeps on that CPU path (python/batchfit/__init__.py:84-86).
 """
 from __future__ import annotations
 
@@ -170,7 +171,11 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
     """Fit shared x[M] and y[N,M]; return params[N,P], status/iterations/chi2[N].
 
     A single y[M] is accepted as N=1. ``device`` is "auto" or "hip"; both run
-    on the GPU in float32 and never fall back to a CPU path. ``p0=None`` runs
+    on the GPU in float32 and never fall back to a CPU path. As in the
+    reference, "auto" takes float32 batches only (the reference keeps float64
+    on its CPU path, so "auto" raises ValueError for float64 or integer
+    inputs here); "hip" converts any real input to float32, as the
+    reference's explicit GPU device does (python/README.md:56-59). ``p0=None`` runs
     the data-only initial guess on the device. ``uncertainties=True`` adds
     covariance[N,P,P], stderr[N,P] and uncertainty_status[N].
 
@@ -192,6 +197,17 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
         raise TypeError("uncertainties must be a boolean")
     if mapping not in _MAPPINGS:
         raise ValueError(f"mapping must be one of {tuple(_MAPPINGS)}")
+    if device == "auto":
+        # python/batchfit/__init__.py:84-86: "auto" sends a float32 batch to
+        # the GPU and keeps any other dtype (float64, integers) on the float64
+        # CPU path, which this package does not include. Converting it here
+        # would change the results' precision and dtype without a word.
+        common = np.result_type(_real_array(x, "x").dtype, _real_array(y, "y").dtype)
+        if common != np.dtype(np.float32):
+            raise ValueError(
+                f"device='auto' fits {common} inputs on the float64 CPU path of the reference, "
+                "which this package does not include; pass float32 arrays, or device='hip' to "
+                "fit in float32 (inputs converted, float32 results)")
     x, y, p0, lower, upper = _validate_batch(x, y, model, p0, bounds)
     dtype = np.dtype(np.float32)
     n, p = y.shape[0], _PARAMETERS[model]

@@ -316,7 +316,7 @@ struct RaggedLayout {
     std::vector<RaggedRun> runs;
     std::vector<size_t> cost_order;  // runs, heaviest first
     bool any_guess = false, any_p0 = false;
-    uint32_t out_rows = 0;
+    uint64_t out_rows = 0;  // 1 + the largest output row (rows are distinct, < 2^32 - 1)
 };
 
 // Device tables of a ragged batch.

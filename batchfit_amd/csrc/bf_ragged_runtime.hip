@@ -102,10 +102,23 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
                    ": model, grid or config out of range, or M differs from its grid";
         if (d.p0_row == 0xffffffffu && d.points > kGuessMaxPoints)
             return "the device initial guess supports M up to 12,288 points; pass p0";
+        if (d.out_row == 0xffffffffu)
+            return "fit " + std::to_string(i) + ": output row 0xffffffff is out of range";
         out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;
         out.any_p0 = out.any_p0 || d.p0_row != 0xffffffffu;
-        out.out_rows = std::max(out.out_rows, d.out_row + 1);
+        out.out_rows = std::max(out.out_rows, uint64_t(d.out_row) + 1);
         bucket[d.model * kRaggedClassCount + ragged_class_of(d.model, d.points)].push_back(i);
+    }
+    // Two fits writing one output row would race on it (and on its guess
+    // row): every fit needs a row of its own.
+    {
+        std::vector<uint32_t> rows(fits);
+        for (uint32_t i = 0; i < fits; ++i)
+            rows[i] = fit_table[i].out_row;
+        std::sort(rows.begin(), rows.end());
+        for (uint32_t i = 1; i < fits; ++i)
+            if (rows[i] == rows[i - 1])
+                return "output row " + std::to_string(rows[i]) + " is named by more than one fit";
     }
     out.order.reserve(fits);
     // Relative work: points × parameters-weighted passes (a P=8 fit costs
@@ -383,9 +396,7 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if (!plan)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "plan pointer is required");
     *plan = nullptr;
-    Runtime &rt = Runtime::get();
-    if (rt.status != BF_OK)
-        return fail(err, errlen, BF_UNAVAILABLE, "HIP fitting is unavailable: " + rt.why);
+    // The tables are checked on the host before any device work.
     if (numerics != 0 && numerics != (BF_COMPENSATED | BF_MIXED_FP64))
         return fail(err, errlen, BF_INVALID_ARGUMENT,
                     "ragged batches support numerics 0 (FP32) or 5 (compensated mixed)");
@@ -397,6 +408,11 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
     if (!why.empty()) {
         delete p;
         return fail(err, errlen, BF_INVALID_ARGUMENT, why);
+    }
+    Runtime &rt = Runtime::get();
+    if (rt.status != BF_OK) {
+        delete p;
+        return fail(err, errlen, BF_UNAVAILABLE, "HIP fitting is unavailable: " + rt.why);
     }
     p->stream = stream ? static_cast<hipStream_t>(stream) : rt.stream;
     p->fits = fits;

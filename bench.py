@@ -173,7 +173,7 @@ def bench_metric(S, args):
         "kernel_ms": {"grid": float(kms[:, 0].mean()), "guess": float(kms[:, 1].mean()),
                       "lm": lm_ms},
         "roofline": {
-            "bound": "mfma", "engine": "VALU FP32 (MFMA unused: P<=8 normal equations)",
+            "bound": "valu_fp32", "engine": "VALU FP32 (MFMA unused: P<=8 normal equations)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(),
             "kernel": (f"{'lm_pool_kernel' if tile == 0 and lanes > 1 else 'lm_kernel'}"

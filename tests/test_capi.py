@@ -144,3 +144,44 @@ def test_plan_follows_the_tuned_table_without_a_device():
     assert plan("lorentzian", 100, mapping="lane") == (1, 0)
     with pytest.raises(ValueError, match="numerics"):
         plan("gaussian", 256, numerics=1, mapping="group16")
+
+
+@pytest.mark.parametrize("rows,message", [
+    ([0, 1, 1], b"more than one fit"), ([0, 0xFFFFFFFF, 2], b"out of range")])
+def test_ragged_plan_rejects_bad_output_rows(rows, message):
+    """A ragged plan needs one output row per fit, below 0xffffffff: a repeated
+    row would make two fits write the same outputs at once, and 0xffffffff
+    would wrap the row count used to size the guess buffer. Both are argument
+    errors found on the host, before any device work."""
+    from batchfit_amd import ragged
+    lib = _native.library()
+    ragged._setup(lib)
+    fits = np.zeros(3, ragged.FIT_DESC)
+    fits["y_offset"] = [0, 32, 64]
+    fits["points"] = 32
+    fits["p0_row"] = ragged.NO_P0
+    fits["out_row"] = rows
+    grids = np.zeros(1, ragged.GRID_DESC)
+    grids["points"] = 32
+    cfg = _native.default_config()
+    handle = ctypes.c_void_p()
+    err = _native.error_buffer()
+    rc = lib.bf_hip_ragged_create(ctypes.byref(handle), 3, fits.ctypes.data, 1, grids.ctypes.data,
+                                  1, ctypes.cast(ctypes.pointer(cfg), ctypes.c_void_p), 5, 0,
+                                  None, err, len(err))
+    assert rc == _native.BF_INVALID_ARGUMENT and message in err.value, (rc, err.value)
+    assert not handle.value
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.int64])
+def test_auto_device_keeps_float64_off_the_gpu(dtype):
+    """python/batchfit/__init__.py:84-86 of the reference: device="auto" runs a
+    float32 batch on the GPU and keeps float64 (and integer) inputs on its
+    float64 CPU path. That path is not in this package, so "auto" refuses such
+    inputs instead of narrowing them; "hip" is the explicit float32 request."""
+    x = np.linspace(-4, 4, 129).astype(dtype)
+    y = np.tile(2 / (1 + ((x - .2) / .7) ** 2), (3, 1)).astype(dtype)
+    with pytest.raises(ValueError, match="device='hip'"):
+        batchfit_amd.fit(x, y)
+    with pytest.raises(ValueError, match="device='hip'"):
+        batchfit_amd.fit(x.astype(np.float32), y, device="auto")
