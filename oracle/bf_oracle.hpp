@@ -371,9 +371,23 @@ template <typename T> bool all_equal_to_first(const T *v, std::size_t m) {
     return true;
 }
 
+// Values of the stop tests of a fit's last iteration (diagnostics for
+// logging why two fits of the same row stopped differently; they do not
+// change the fit): the gradient norm and its tolerance, the step norm and its
+// threshold, |χ² reduction| of the last trial with the cost threshold, the
+// predicted reduction with its threshold, the smallest width·factor over the
+// grid spacing at the returned parameters (below 1: unresolved peak), the
+// final λ and which test ended the fit (0 none, 1 gradient, 2 step, 3 cost).
+struct Stops {
+    double gradient = NAN, gradient_tolerance = NAN, step = NAN, step_threshold = NAN;
+    double reduction = NAN, cost_threshold = NAN, predicted = NAN, predicted_threshold = NAN;
+    double width_over_spacing = NAN, lambda = NAN, fired = 0;
+};
+
 // One bounded LM fit (cpu/solver.hpp:196-342).
 template <typename T, int MODEL>
-Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Options<T> &o) {
+Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Options<T> &o,
+               Stops *stops = nullptr) {
     constexpr int P = param_count<MODEL>();
     Fit<T> out;
     for (int k = 0; k < 8; ++k)
@@ -425,7 +439,14 @@ Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Optio
             if (!pinned[k])
                 gnorm = hi_of(gnorm, std::abs(g[k]));
         }
+        if (stops) {
+            stops->gradient = double(gnorm);
+            stops->gradient_tolerance = double(o.gradient_tolerance);
+            stops->lambda = double(lambda);
+        }
         if (gnorm <= o.gradient_tolerance) {
+            if (stops)
+                stops->fired = 1;
             out.status = resolved<T, MODEL>(out.params, spacing);
             return out;
         }
@@ -459,8 +480,15 @@ Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Optio
             step[k] = trial[k] - out.params[k];
             param_norm = hi_of(param_norm, std::abs(out.params[k]));
         }
+        if (stops) {
+            stops->step = double(step_norm);
+            stops->step_threshold =
+                double(o.step_tolerance * (param_norm + o.step_tolerance) / hi_of(T(1), lambda));
+        }
         if (step_norm <=
             o.step_tolerance * (param_norm + o.step_tolerance) / hi_of(T(1), lambda)) {
+            if (stops)
+                stops->fired = 2;
             out.status = resolved<T, MODEL>(out.params, spacing);
             return out;
         }
@@ -477,6 +505,12 @@ Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Optio
         const bool small_change = std::isfinite(trial_chi2) && std::isfinite(predicted) &&
                                   predicted >= T(0) && std::abs(reduction) <= threshold &&
                                   predicted <= threshold / hi_of(T(1), lambda);
+        if (stops) {
+            stops->reduction = double(std::abs(reduction));
+            stops->cost_threshold = double(threshold);
+            stops->predicted = double(predicted);
+            stops->predicted_threshold = double(threshold / hi_of(T(1), lambda));
+        }
         if (std::isfinite(trial_chi2) && reduction > T(0) && std::isfinite(predicted) &&
             predicted > T(0)) {
             for (int k = 0; k < P; ++k)
@@ -490,11 +524,27 @@ Fit<T> fit_one(const T *x, const T *y, std::size_t m, const T *init, const Optio
             lambda = lo_of(lambda_cap, lambda * T(10));
         }
         if (small_change) {
+            if (stops)
+                stops->fired = 3;
             out.status = resolved<T, MODEL>(out.params, spacing);
             return out;
         }
     }
     return out;
+}
+
+// Smallest width·factor over the spacing of a spectral model's peaks (the
+// quantity of resolved(): below 1 means unresolved_peak).
+template <typename T, int MODEL> double width_ratio(const T *p, T spacing) {
+    if constexpr (!spectral<MODEL>()) {
+        return NAN;
+    } else {
+        const double f = double(width_factor<T, MODEL>());
+        double r = std::abs(double(p[2])) * f / double(spacing);
+        if constexpr (param_count<MODEL>() == 8)
+            r = std::min(r, std::abs(double(p[5])) * f / double(spacing));
+        return r;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,11 @@ PARAMS = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8
           "constant": 1, "exponential": 1, "affine": 2, "line": 2, "duplicate": 2}
 
 
+STOP_FIELDS = ("gradient", "gradient_tolerance", "step", "step_threshold", "reduction",
+               "cost_threshold", "predicted", "predicted_threshold", "width_over_spacing",
+               "lambda", "fired")
+
+
 class Options(ctypes.Structure):
     _fields_ = [
         ("max_iterations", ctypes.c_uint32),
@@ -108,6 +113,34 @@ class Oracle:
             fn = getattr(self.lib, f"{self.prefix}uncertainty_{suffix}")
             fn.argtypes = [i32, vp, sz, vp, ctypes.c_int32, scalar, vp, vp, vp, vp, vp]
             fn.restype = i32
+
+    def fit_stops(self, x, y, model, p0=None, bounds=None, *, dtype=None, options=None,
+                  threads=1):
+        """The values of each row's last stop tests (restatement only), an
+        [N, 11] float64 array with the columns of STOP_FIELDS."""
+        if self.kind != "port":
+            raise ValueError("stop diagnostics exist in the restatement only")
+        dtype = np.dtype(dtype or np.result_type(np.asarray(x).dtype, np.asarray(y).dtype))
+        x = np.ascontiguousarray(x, dtype=dtype)
+        y = np.ascontiguousarray(y, dtype=dtype)
+        if y.ndim == 1:
+            y = y.reshape(1, -1)
+        n, m = y.shape
+        p = PARAMS[model]
+        if p0 is not None:
+            p0 = np.ascontiguousarray(np.broadcast_to(np.asarray(p0, dtype=dtype), (n, p)))
+        lower, upper = _bounds(bounds, p, dtype)
+        out = np.empty((n, 11), np.float64)
+        opts = options if options is not None else make_options()
+        fn = getattr(self.lib, f"oracle_fit_stops_{self._suffix(dtype)}")
+        fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                       ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.POINTER(Options), ctypes.c_int, ctypes.c_void_p]
+        fn.restype = ctypes.c_int
+        if fn(MODELS[model], _ptr(x), m, _ptr(y), n, _ptr(p0), _ptr(lower), _ptr(upper),
+              ctypes.byref(opts), int(threads), _ptr(out)) != 0:
+            raise ValueError("stop diagnostics exist for the spectral models only")
+        return out
 
     @staticmethod
     def _suffix(dtype):

@@ -67,6 +67,19 @@ typedef struct {
 BF_ORACLE_DECLARE(oracle_)
 BF_ORACLE_DECLARE(ref_)
 
+/* Restatement only: the fits of oracle_fit_* with the values of each row's
+ * last stop tests, 11 doubles per row: gradient norm, gradient tolerance,
+ * step norm, step threshold, |chi2 reduction|, cost threshold, predicted
+ * reduction, predicted threshold, smallest width*factor/spacing at the
+ * result, final lambda, test that ended the fit (0 none, 1 gradient, 2 step,
+ * 3 cost). Diagnostics for logging status disagreements with their margins. */
+int oracle_fit_stops_f32(int model, const float *x, size_t m, const float *y, size_t n,
+                         const float *p0, const float *lower, const float *upper,
+                         const bf_oracle_options *options, int threads, double *stops);
+int oracle_fit_stops_f64(int model, const double *x, size_t m, const double *y, size_t n,
+                         const double *p0, const double *lower, const double *upper,
+                         const bf_oracle_options *options, int threads, double *stops);
+
 /* Restatement only: Kahan accumulation of n terms (numerics.hpp:12-18). */
 float oracle_kahan_sum_f32(const float *terms, size_t n, float start);
 

@@ -153,6 +153,59 @@ int fit_any(int model, const T *x, std::size_t m, const T *y, std::size_t n, con
     return -1;
 }
 
+// Fits (from p0 or the guess) with the values of their last stop tests,
+// 11 doubles per row in the order of struct Stops.
+template <typename T, int MODEL>
+void stops_rows(const T *x, std::size_t m, const T *y, std::size_t n, const T *p0,
+                const Options<T> &o, int threads, double *out) {
+    constexpr int P = param_count<MODEL>();
+    const T spacing = min_spacing(x, m);
+    chunked(n, threads, [&](std::size_t i) {
+        const T *row = y + i * m;
+        T init[8];
+        if (p0) {
+            for (int k = 0; k < P; ++k)
+                init[k] = p0[i * P + k];
+        } else {
+            const Guess<T> g = guess_one<T, MODEL>(x, row, m);
+            for (int k = 0; k < P; ++k)
+                init[k] = g.params[k];
+        }
+        Stops st;
+        const Fit<T> f = fit_one<T, MODEL>(x, row, m, init, o, &st);
+        st.width_over_spacing = width_ratio<T, MODEL>(f.params, spacing);
+        const double v[11] = {st.gradient, st.gradient_tolerance, st.step, st.step_threshold,
+                              st.reduction, st.cost_threshold, st.predicted,
+                              st.predicted_threshold, st.width_over_spacing, st.lambda, st.fired};
+        for (int k = 0; k < 11; ++k)
+            out[i * 11 + k] = v[k];
+    });
+}
+
+template <typename T>
+int stops_any(int model, const T *x, std::size_t m, const T *y, std::size_t n, const T *p0,
+              const T *lower, const T *upper, const bf_oracle_options *opt, int threads,
+              double *out) {
+    switch (model) {
+    case GAUSSIAN:
+        stops_rows<T, GAUSSIAN>(x, m, y, n, p0, make_options<T>(opt, lower, upper, 3), threads, out);
+        return 0;
+    case LORENTZIAN:
+        stops_rows<T, LORENTZIAN>(x, m, y, n, p0, make_options<T>(opt, lower, upper, 3), threads,
+                                  out);
+        return 0;
+    case PSEUDO_VOIGT:
+        stops_rows<T, PSEUDO_VOIGT>(x, m, y, n, p0, make_options<T>(opt, lower, upper, 4), threads,
+                                    out);
+        return 0;
+    case TWO_LORENTZIAN:
+        stops_rows<T, TWO_LORENTZIAN>(x, m, y, n, p0, make_options<T>(opt, lower, upper, 8),
+                                      threads, out);
+        return 0;
+    }
+    return -1;
+}
+
 template <typename T, int MODEL>
 void guess_rows(const T *x, std::size_t m, const T *y, std::size_t n, int threads, T *params,
                 T *baseline, int32_t *status) {
@@ -268,6 +321,16 @@ int oracle_fit_f64(int model, const double *x, size_t m, const double *y, size_t
                    double *cov, double *stderr_, int32_t *ustatus) {
     return fit_any<double>(model, x, m, y, n, p0, lower, upper, options, uncertainties, threads,
                            params, status, iterations, chi2, cov, stderr_, ustatus);
+}
+int oracle_fit_stops_f32(int model, const float *x, size_t m, const float *y, size_t n,
+                         const float *p0, const float *lower, const float *upper,
+                         const bf_oracle_options *options, int threads, double *stops) {
+    return stops_any<float>(model, x, m, y, n, p0, lower, upper, options, threads, stops);
+}
+int oracle_fit_stops_f64(int model, const double *x, size_t m, const double *y, size_t n,
+                         const double *p0, const double *lower, const double *upper,
+                         const bf_oracle_options *options, int threads, double *stops) {
+    return stops_any<double>(model, x, m, y, n, p0, lower, upper, options, threads, stops);
 }
 int oracle_guess_f32(int model, const float *x, size_t m, const float *y, size_t n, int threads,
                      float *params, float *baseline, int32_t *status) {
