@@ -192,10 +192,38 @@ int check_common(uint32_t model, const bf_config *cfg, char *err, size_t errlen)
     return BF_OK;
 }
 
+// Scratch of the no-allocation entry (bf_hip_fit_device_noalloc): grid facts
+// and the retiring kernel's work counter in the first 512 bytes, then the
+// device guess rows (N·P floats). Every GPU the library runs on holds its own
+// copy until the process ends; calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocGuessBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[512 + kNoAllocGuessBytes];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's name is a host-side stand-in, not a device address).
+int noalloc_scratch(char **out, char *err, size_t errlen) {
+    static std::mutex mu;
+    static char *address[64] = {};
+    int device = 0;
+    BF_TRY(hipGetDevice(&device), "hipGetDevice");
+    if (device < 0 || device >= 64)
+        return fail(err, errlen, BF_RUNTIME_ERROR, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!address[device]) {
+        void *ptr = nullptr;
+        BF_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        address[device] = static_cast<char *>(ptr);
+    }
+    *out = address[device];
+    return BF_OK;
+}
+
+// fixed: null → scratch is taken stream-ordered from the device pool;
+// otherwise the no-allocation scratch above (no transpose, no timing events).
 int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, const float *x,
                const float *y, const float *p0, float *params, int32_t *status,
                uint32_t *iterations, float *chi2, float *cov, float *se, int32_t *us, char *err,
-               size_t errlen) {
+               size_t errlen, char *fixed = nullptr) {
     if (const int rc = check_common(model, cfg, err, errlen))
         return rc;
     const uint32_t n = cfg->fits, m = cfg->points;
@@ -213,15 +241,28 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         return rc;
     hipStream_t stream =
         launch && launch->stream ? static_cast<hipStream_t>(launch->stream) : Runtime::get().stream;
-    const bool timing = g_timing.load(std::memory_order_relaxed);
+    if (fixed && !p0 && size_t(n) * P * sizeof(float) > kNoAllocGuessBytes)
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "the no-allocation entry holds device guesses for at most " +
+                        std::to_string(kNoAllocGuessBytes / (P * sizeof(float))) +
+                        " fits of this model; pass p0 or use bf_hip_fit_device");
+    if (fixed && plan.G == 1 && launch && launch->point_major)
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "the no-allocation entry does not transpose y to point-major");
+    const bool timing = !fixed && g_timing.load(std::memory_order_relaxed);
     Timing &tm = t_timing;
     tm.recorded = false;
     for (bool &b : tm.phase)
         b = false;
-    Scratch scratch(stream);
+    Scratch temp(stream);
     GridInfo *grid = nullptr;
-    // GridInfo, then the work counter of the retiring kernel at a 256-byte offset.
-    BF_TRY(scratch.get(reinterpret_cast<void **>(&grid), 512), "hipMallocAsync(grid)");
+    // GridInfo, then the work counter of the retiring kernel at a 256-byte
+    // offset; grid_kernel writes both (the counter is cleared by the kernel,
+    // not by a memset).
+    if (fixed)
+        grid = reinterpret_cast<GridInfo *>(fixed);
+    else
+        BF_TRY(temp.get(reinterpret_cast<void **>(&grid), 512), "hipMallocAsync(grid)");
     static_assert(sizeof(GridInfo) <= 256, "grid facts fit before the counter");
     uint32_t *pool = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(grid) + 256);
     if (timing)
@@ -240,8 +281,11 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
             return fail(err, errlen, BF_INVALID_ARGUMENT,
                         "the device initial guess supports M up to 12,288 points; pass p0");
         float *guess = nullptr;
-        BF_TRY(scratch.get(reinterpret_cast<void **>(&guess), size_t(n) * P * sizeof(float)),
-               "hipMallocAsync(guess)");
+        if (fixed)
+            guess = reinterpret_cast<float *>(fixed + 512);  // every row written by the guess
+        else
+            BF_TRY(temp.get(reinterpret_cast<void **>(&guess), size_t(n) * P * sizeof(float)),
+                   "hipMallocAsync(guess)");
         GuessArgs ga{x, y, (long long)m, 1, grid, n, m, m, sort_n, guess, nullptr, nullptr};
         BF_TRY(launch_guess(model, ga, lds, stream), "initial guess launch");
         init = guess;
@@ -254,7 +298,7 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     long long ys_fit = m, ys_pt = 1;
     if (plan.G == 1 && launch && launch->point_major) {
         float *yt = nullptr;
-        BF_TRY(scratch.get(reinterpret_cast<void **>(&yt), size_t(n) * m * sizeof(float)),
+        BF_TRY(temp.get(reinterpret_cast<void **>(&yt), size_t(n) * m * sizeof(float)),
                "hipMallocAsync(point-major y)");
         hipLaunchKernelGGL(transpose_kernel, dim3((n + 63) / 64, (m + 63) / 64), dim3(256), 0,
                            stream, y, yt, n, m);
@@ -375,6 +419,20 @@ int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *lau
                       float *d_stderr, int32_t *d_ustatus, char *err, size_t errlen) {
     return fit_device(model, cfg, launch, d_x, d_y, d_p0, d_params, d_status, d_iterations,
                       d_chi2, d_cov, d_stderr, d_ustatus, err, errlen);
+}
+
+int bf_hip_fit_device_noalloc(uint32_t model, const bf_config *cfg, const bf_launch *launch,
+                              const float *d_x, const float *d_y, const float *d_p0,
+                              float *d_params, int32_t *d_status, uint32_t *d_iterations,
+                              float *d_chi2, float *d_cov, float *d_stderr, int32_t *d_ustatus,
+                              char *err, size_t errlen) {
+    if (const int rc = check_common(model, cfg, err, errlen))
+        return rc;
+    char *fixed = nullptr;
+    if (const int rc = noalloc_scratch(&fixed, err, errlen))
+        return rc;
+    return fit_device(model, cfg, launch, d_x, d_y, d_p0, d_params, d_status, d_iterations,
+                      d_chi2, d_cov, d_stderr, d_ustatus, err, errlen, fixed);
 }
 
 int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, const float *x,

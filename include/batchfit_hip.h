@@ -125,6 +125,22 @@ int bf_hip_fit_device(uint32_t model, const bf_config *cfg, const bf_launch *lau
                       int32_t *d_status, uint32_t *d_iterations, float *d_chi2, float *d_cov,
                       float *d_stderr, int32_t *d_ustatus, char *err, size_t errlen);
 
+/* bf_hip_fit_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same arguments and results as bf_hip_fit_device; its scratch (grid facts,
+ * work counter, device-guess rows) is one static device array per GPU that
+ * holds the guesses of at most 32 MiB / (4·P) fits (2,097,152 of P=4) when p0
+ * is NULL, and any N when p0 is given. Larger N, and the point-major
+ * transpose of BF_MAP_LANE, are rejected with BF_INVALID_ARGUMENT; no kernel
+ * timing is recorded. Calls that overlap on one GPU share that array and race:
+ * serialise them (one stream, or events). */
+int bf_hip_fit_device_noalloc(uint32_t model, const bf_config *cfg, const bf_launch *launch,
+                              const float *d_x, const float *d_y, const float *d_p0,
+                              float *d_params, int32_t *d_status, uint32_t *d_iterations,
+                              float *d_chi2, float *d_cov, float *d_stderr, int32_t *d_ustatus,
+                              char *err, size_t errlen);
+
 /* Ragged batches: fits of any model and M, each with its own config row, in
  * one hipGraph launch (persistent kernels fed by device work queues).
  * bf_hip_ragged_create takes HOST descriptor tables, validates them, buckets
