@@ -250,10 +250,23 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
                    const bf_config *cfg, const bf_npy_options *options, bf_npy_consumer consume,
                    void *user, bf_npy_stats *stats, char *err, size_t errlen);
 
+/* Noise of bf_hip_synthesize. The SNR modes are those of the reference's
+ * generator (src/cpu/synthetic.cpp:83-108): Gaussian with σ = RMS of the
+ * clean row / snr, and Poisson counts y = Poisson(gain·f)/gain with gain =
+ * snr²·Σf/Σf² (a row whose clean signal is negative anywhere, or a point
+ * whose expected count exceeds 1e12, is written as NaN where the reference
+ * raises). */
+enum bf_noise {
+    BF_NOISE_ABSOLUTE = 0,     /* σ = noise */
+    BF_NOISE_RELATIVE = 1,     /* σ = noise · amplitude (parameter 0) */
+    BF_NOISE_SNR_GAUSSIAN = 2, /* noise = snr */
+    BF_NOISE_SNR_POISSON = 3   /* noise = snr */
+};
+
 /* Seeded synthetic spectra written straight into HBM (benchmarks, serving
  * load): y[n*m] (fit-major) of `model` on the device grid d_x[m], parameters
- * per fit drawn uniformly from [lower[k], upper[k]) (8 entries), noise
- * N(0, σ²) with σ = noise·amplitude (noise_relative) or σ = noise. layout 1
+ * per fit drawn uniformly from [lower[k], upper[k]) (8 entries), noise by
+ * noise_mode (bf_noise; 0 with noise = 0 is the noise-free signal). layout 1
  * (two_lorentzian) draws one HWHM for both peaks and sets c2 = c1 +
  * U[sep_lo, sep_hi)·HWHM. Philox4x32-10 counters (seed, fit_offset + fit,
  * point) make every fit independent of how the batch is split. d_truth[n*8]
@@ -262,7 +275,7 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
  * src/cpu/synthetic.cpp on the host). */
 int bf_hip_synthesize(uint32_t model, uint32_t n, uint32_t m, const float *d_x,
                       const float *lower, const float *upper, uint32_t layout, float sep_lo,
-                      float sep_hi, uint32_t noise_relative, float noise, uint64_t seed,
+                      float sep_hi, uint32_t noise_mode, float noise, uint64_t seed,
                       uint64_t fit_offset, float *d_y, float *d_truth, void *stream, char *err,
                       size_t errlen);
 

@@ -42,3 +42,52 @@ def synth_parameters(n, lower, upper, seed, fit_offset=0, layout=0, sep=(2.0, 6.
         s = np.float32(sep[0]) + (np.float32(sep[1]) - np.float32(sep[0])) * unit(w[:, 8])
         p[:, 4] = p[:, 1] + s * p[:, 2]
     return p.astype(np.float32)
+
+
+def unit53(hi, lo):
+    """Uniform in (0, 1) with 53 bits from two words (float64)."""
+    return (float(((int(hi) >> 5) << 26) | (int(lo) >> 6)) + 0.5) * 2.0 ** -53
+
+
+def _words(fit, j, d, seed):
+    ctr = np.array([[fit & 0xFFFFFFFF, fit >> 32, j, d]], np.uint32)
+    return philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32))[0]
+
+
+def poisson_count(mu, fit, j, seed):
+    """The device's Poisson count for the stream (seed, fit, point j): inversion
+    by sequential search below a mean of 10, PTRS (W. Hörmann, Insurance:
+    Mathematics and Economics 12 (1993) 39-45) from 10; try t of PTRS reads
+    counter (fit, fit>>32, j, 8 + t)."""
+    import math
+    if not mu > 0:
+        return 0.0
+    if mu < 10:
+        w = _words(fit, j, 8, seed)
+        u = unit53(w[0], w[1])
+        p = math.exp(-mu)
+        f, k = p, 0.0
+        while u > f and k < 1000:
+            k += 1.0
+            p *= mu / k
+            f += p
+        return k
+    smu, lmu = math.sqrt(mu), math.log(mu)
+    b = 0.931 + 2.53 * smu
+    a = -0.059 + 0.02483 * b
+    inv_alpha = 1.1239 + 1.1328 / (b - 3.4)
+    vr = 0.9277 - 3.6224 / (b - 2.0)
+    for t in range(256):
+        w = _words(fit, j, 8 + t, seed)
+        u = unit53(w[0], w[1]) - 0.5
+        v = unit53(w[2], w[3])
+        us = 0.5 - abs(u)
+        k = math.floor((2.0 * a / us + b) * u + mu + 0.43)
+        if us >= 0.07 and v <= vr:
+            return float(k)
+        if k < 0 or (us < 0.013 and v > us):
+            continue
+        if math.log(v) + math.log(inv_alpha) - math.log(a / (us * us) + b) <= \
+                -mu + k * lmu - math.lgamma(k + 1.0):
+            return float(k)
+    return mu
