@@ -35,3 +35,13 @@ def test_facade_fit_and_submit_on_caller_stream(tmp_path):
     exe = build(tmp_path)
     out = subprocess.run([exe, "device"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0 and "facade ok (device)" in out.stdout, out.stdout + out.stderr
+
+
+def test_coordinates_header(tmp_path):
+    """include/batchfit/coordinates.hpp compiled and run on the host."""
+    exe = str(tmp_path / "coordinates_check")
+    subprocess.run(["g++", "-std=c++20", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "coordinates_check.cpp"), "-o", exe,
+                    "-L", LIB, "-lbatchfit_hip", f"-Wl,-rpath,{LIB}"], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "coordinates ok" in out.stdout, out.stdout + out.stderr
