@@ -142,13 +142,17 @@ def _make_config(n, m, lower, upper, max_iterations, numerics):
 _MAPPINGS = {"auto": (0, 0, 0), "lane": (1, 0, 0), "group8": (2, 8, 0), "group16": (2, 16, 0),
              "group32": (2, 32, 0), "group64": (2, 64, 0), "wave": (2, 64, 0),
              "group4_streamed": (2, 4, 1), "group8_streamed": (2, 8, 1), "group16_streamed": (2, 16, 1),
-             "group32_streamed": (2, 32, 1), "group64_streamed": (2, 64, 1)}
+             "group32_streamed": (2, 32, 1), "group64_streamed": (2, 64, 1),
+             "group128": (2, 128, 0), "group256": (2, 256, 0), "workgroup": (2, 256, 0),
+             "group128_streamed": (2, 128, 1), "group256_streamed": (2, 256, 1),
+             "latency": (3, 0, 0)}
 
 
-def plan(model, points, *, numerics=None, mapping="auto"):
+def plan(model, points, *, numerics=None, mapping="auto", fits=0):
     """The mapping fit() uses for ``points`` per spectrum: ``(lanes, tile)``,
     lanes per fit (1 = one lane per fit) and points per lane held in
-    registers (0 = read from memory on every pass). Host-only: no device is
+    registers (0 = read from memory on every pass). ``fits`` is the batch
+    size, which only mapping="latency" depends on. Host-only: no device is
     needed and nothing is launched."""
     if model not in _PARAMETERS:
         raise ValueError(f"unknown model {model!r}; choose one of {tuple(_PARAMETERS)}")
@@ -159,10 +163,10 @@ def plan(model, points, *, numerics=None, mapping="auto"):
     launch.mapping, launch.group_lanes, launch.streamed = _MAPPINGS[mapping]
     lanes, tile = ctypes.c_uint32(), ctypes.c_uint32()
     err = _native.error_buffer()
-    _native.check(lib.bf_hip_plan(_native.MODELS[model], int(points),
-                                  NUMERICS_DEFAULT if numerics is None else int(numerics),
-                                  ctypes.byref(launch), ctypes.byref(lanes), ctypes.byref(tile),
-                                  err, len(err)), err)
+    _native.check(lib.bf_hip_plan_batch(_native.MODELS[model], int(points), int(fits),
+                                        NUMERICS_DEFAULT if numerics is None else int(numerics),
+                                        ctypes.byref(launch), ctypes.byref(lanes),
+                                        ctypes.byref(tile), err, len(err)), err)
     return lanes.value, tile.value
 
 
@@ -181,9 +185,12 @@ def fit(x, y, model="lorentzian", p0=None, bounds=None, device="auto", *, uncert
 
     Keyword-only extensions of the HIP device: ``numerics`` (bit0 compensated
     partials, bit1 scaled Cholesky, bit2 FP64 combine/solve; default 5),
-    ``mapping`` ("auto", "lane", "group8", "group16", "group32", "group64"; a "_streamed"
-    suffix reads the points from memory on every pass instead of registers,
-    with the same results; "group4_streamed" streams 4-lane groups) and
+    ``mapping`` ("auto", "lane", "group8", "group16", "group32", "group64", and
+    "group128" / "group256" ("workgroup"), a workgroup of 2 or 4 waves per fit;
+    a "_streamed" suffix reads the points from memory on every pass instead of
+    registers, with the same results; "group4_streamed" streams 4-lane groups;
+    "latency" gives a batch too small to fill the device 64, 128 or 256 lanes
+    per fit, for a shorter call, and is "auto" otherwise) and
     ``max_iterations`` (default 100, cpu/solver.hpp:27).
     """
     if model not in _PARAMETERS:

@@ -17,13 +17,13 @@ PARAMETERS = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian
 
 BF_OK, BF_INVALID_ARGUMENT, BF_RUNTIME_ERROR, BF_UNAVAILABLE, BF_TIMEOUT = 0, 1, 2, 3, 4
 COMPENSATED, SCALED_CHOLESKY, MIXED_FP64 = 1, 2, 4
-MAP_AUTO, MAP_LANE, MAP_GROUP = 0, 1, 2
+MAP_AUTO, MAP_LANE, MAP_GROUP, MAP_LATENCY = 0, 1, 2, 3
 
 # Every symbol include/batchfit_hip.h declares.
 EXPORTS = (
     "bf_hip_abi_version", "bf_hip_available", "bf_hip_default_config", "bf_hip_fit",
     "bf_hip_fit_device", "bf_hip_fit_device_noalloc", "bf_hip_guess", "bf_hip_model_eval",
-    "bf_hip_plan", "bf_hip_set_timing",
+    "bf_hip_plan", "bf_hip_plan_batch", "bf_hip_set_timing",
     "bf_hip_last_timing", "bf_hip_malloc", "bf_hip_free", "bf_hip_memcpy", "bf_hip_synchronize",
     "bf_hip_ragged_create", "bf_hip_ragged_run", "bf_hip_ragged_timing", "bf_hip_ragged_destroy",
     "bf_hip_synthesize", "bf_serve_create", "bf_serve_submit", "bf_serve_wait", "bf_serve_stats",
@@ -101,6 +101,9 @@ def library() -> ctypes.CDLL:
     lib.bf_hip_plan.argtypes = [u32, u32, u32, vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                 ctypes.c_char_p, sz]
     lib.bf_hip_plan.restype = i32
+    lib.bf_hip_plan_batch.argtypes = [u32, u32, u32, u32, vp, ctypes.POINTER(u32),
+                                      ctypes.POINTER(u32), ctypes.c_char_p, sz]
+    lib.bf_hip_plan_batch.restype = i32
     lib.bf_hip_set_timing.argtypes = [i32]
     lib.bf_hip_set_timing.restype = i32
     lib.bf_hip_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), i32]

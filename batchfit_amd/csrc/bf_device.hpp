@@ -279,10 +279,54 @@ template <int G, int T> constexpr int scatter_slots() {
 }
 }  // namespace detail
 
+// Groups of G = 64·W lanes (W = 2 or 4 waves) own one fit per workgroup of
+// G threads (the threadgroup-per-fit form, src/metal/cooperative.metal:16-32,
+// 338-349). A reduction runs within each wave first; the waves' results,
+// identical in every lane of a wave, are then combined through LDS in wave
+// order, so every lane of the workgroup holds the same bits. Each call is one
+// barrier pair; the whole workgroup takes part in every call.
+template <int G, typename V, typename Op>
+__device__ __forceinline__ V across_waves(V v, Op op) {
+    static_assert(G == 128 || G == 256, "workgroup groups are 2 or 4 waves");
+    __shared__ V slot[G / 64];
+    if ((threadIdx.x & 63) == 0)
+        slot[threadIdx.x >> 6] = v;
+    __syncthreads();
+    V r = slot[0];
+#pragma unroll
+    for (int w = 1; w < G / 64; ++w)
+        r = op(r, slot[w]);
+    __syncthreads();
+    return r;
+}
+template <int G, int T> __device__ __forceinline__ void across_waves_sum(double (&v)[T]) {
+    static_assert(G == 128 || G == 256, "workgroup groups are 2 or 4 waves");
+    __shared__ double slot[G / 64][T];
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            slot[threadIdx.x >> 6][t] = v[t];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        double r = slot[0][t];
+#pragma unroll
+        for (int w = 1; w < G / 64; ++w)
+            r += slot[w][t];
+        v[t] = r;
+    }
+    __syncthreads();
+}
+
 template <int G, int T>
 __device__ __forceinline__ void group_allreduce_d(double (&v)[T], int gl) {
     if constexpr (G == 1)
         return;
+    if constexpr (G > 64) {
+        group_allreduce_d<64, T>(v, gl & 63);
+        across_waves_sum<G, T>(v);
+        return;
+    }
     detail::scatter_steps<G, 1, T, T>(v, gl);
     constexpr int S = detail::scatter_slots<G, T>();
     double part[S];
@@ -320,6 +364,8 @@ template <int STEP> __device__ __forceinline__ double xlane_d(double v) {
 
 // Sum over the G-lane group in FP32 (plain form).
 template <int G> __device__ __forceinline__ float group_sum_f(float v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_sum_f<64>(v), [](float a, float b) { return a + b; });
     v += xlane_f<1>(v);
     v += xlane_f<2>(v);
     if constexpr (G >= 8)
@@ -334,6 +380,8 @@ template <int G> __device__ __forceinline__ float group_sum_f(float v) {
 }
 // Sum over the G-lane group in FP64 (combination of FP32 compensated partials).
 template <int G> __device__ __forceinline__ double group_sum_d(double v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_sum_d<64>(v), [](double a, double b) { return a + b; });
     v += xlane_d<1>(v);
     v += xlane_d<2>(v);
     if constexpr (G >= 8)
@@ -348,6 +396,8 @@ template <int G> __device__ __forceinline__ double group_sum_d(double v) {
 }
 // Bitwise OR / integer max / min over the group (validation flags, indices).
 template <int G> __device__ __forceinline__ int group_or(int v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_or<64>(v), [](int a, int b) { return a | b; });
     v |= xlane<1>(v);
     v |= xlane<2>(v);
     if constexpr (G >= 8)
@@ -363,6 +413,8 @@ template <int G> __device__ __forceinline__ int group_or(int v) {
 // Group maximum of floats (NaN ignored where one operand is a number, as
 // fmaxf); group_min_f below is its minimum counterpart.
 template <int G> __device__ __forceinline__ float group_max_f(float v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_max_f<64>(v), [](float a, float b) { return fmaxf(a, b); });
     v = fmaxf(v, xlane_f<1>(v));
     v = fmaxf(v, xlane_f<2>(v));
     if constexpr (G >= 8)
@@ -376,6 +428,8 @@ template <int G> __device__ __forceinline__ float group_max_f(float v) {
     return v;
 }
 template <int G> __device__ __forceinline__ int group_max_i(int v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_max_i<64>(v), [](int a, int b) { return max(a, b); });
     v = max(v, xlane<1>(v));
     v = max(v, xlane<2>(v));
     if constexpr (G >= 8)
@@ -389,6 +443,8 @@ template <int G> __device__ __forceinline__ int group_max_i(int v) {
     return v;
 }
 template <int G> __device__ __forceinline__ int group_min_i(int v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_min_i<64>(v), [](int a, int b) { return min(a, b); });
     v = min(v, xlane<1>(v));
     v = min(v, xlane<2>(v));
     if constexpr (G >= 8)
@@ -402,6 +458,8 @@ template <int G> __device__ __forceinline__ int group_min_i(int v) {
     return v;
 }
 template <int G> __device__ __forceinline__ float group_min_f(float v) {
+    if constexpr (G > 64)
+        return across_waves<G>(group_min_f<64>(v), [](float a, float b) { return fminf(a, b); });
     v = fminf(v, xlane_f<1>(v));
     v = fminf(v, xlane_f<2>(v));
     if constexpr (G >= 8)

@@ -17,6 +17,13 @@ template <int MODEL, int G, int R, bool COMP, bool MIXED>
 hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
     if constexpr (!plan_instantiated(G, R, COMP, MIXED)) {
         return hipErrorInvalidValue;
+    } else if constexpr (G > 64) {
+        // One fit per workgroup at a time; at most 2^20 workgroups, each
+        // taking every gridDim-th fit after its first.
+        const uint32_t groups = std::min<uint32_t>(args.n, 1u << 20);
+        hipLaunchKernelGGL((lm_block_kernel<MODEL, G, R, COMP, MIXED>), dim3(groups), dim3(G), 0,
+                           stream, args);
+        return hipGetLastError();
     } else {
         constexpr int per_block = 256 / G;
         const uint32_t blocks = (args.n + per_block - 1) / per_block;
@@ -53,18 +60,30 @@ hipError_t by_tile(const Plan &plan, const FitArgs &args, hipStream_t stream) {
     switch (plan.R) {
     case 0:
         return go<MODEL, G, 0, COMP, MIXED>(plan, args, stream);
-    case 2:
-        return go<MODEL, G, 2, COMP, MIXED>(plan, args, stream);
-    case 4:
-        return go<MODEL, G, 4, COMP, MIXED>(plan, args, stream);
-    case 8:
-        return go<MODEL, G, 8, COMP, MIXED>(plan, args, stream);
-    case 16:
-        return go<MODEL, G, 16, COMP, MIXED>(plan, args, stream);
-    case 32:
-        return go<MODEL, G, 32, COMP, MIXED>(plan, args, stream);
     }
-    return hipErrorInvalidValue;
+    if constexpr (G > 64) {
+        switch (plan.R) {
+        case 8:
+            return go<MODEL, G, 8, COMP, MIXED>(plan, args, stream);
+        case 32:
+            return go<MODEL, G, 32, COMP, MIXED>(plan, args, stream);
+        }
+        return hipErrorInvalidValue;
+    } else {
+        switch (plan.R) {
+        case 2:
+            return go<MODEL, G, 2, COMP, MIXED>(plan, args, stream);
+        case 4:
+            return go<MODEL, G, 4, COMP, MIXED>(plan, args, stream);
+        case 8:
+            return go<MODEL, G, 8, COMP, MIXED>(plan, args, stream);
+        case 16:
+            return go<MODEL, G, 16, COMP, MIXED>(plan, args, stream);
+        case 32:
+            return go<MODEL, G, 32, COMP, MIXED>(plan, args, stream);
+        }
+        return hipErrorInvalidValue;
+    }
 }
 }  // namespace
 

@@ -121,8 +121,9 @@ inline int tuned_group(int model, uint32_t m) { return tuned_mapping(model, m).l
 
 // Instantiated plans: lane-per-fit with every numerics form; 64-lane groups
 // with tiles 0 (stream), 2 … 32 and every numerics form; 8-, 16- and 32-lane
-// groups streamed or with tiles 2 … 32, and 4-lane groups streamed, in the
-// plain FP32 and compensated-mixed forms.
+// groups streamed or with tiles 2 … 32, 4-lane groups streamed, and workgroups
+// of 128 or 256 lanes streamed or with tiles of 8 or 32 points, in the plain
+// FP32 and compensated-mixed forms.
 constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
     const bool tile = R == 2 || R == 4 || R == 8 || R == 16 || R == 32;
     if (G == 1)
@@ -133,7 +134,17 @@ constexpr bool plan_instantiated(int G, int R, bool comp, bool mixed) {
         return R == 0 && comp == mixed;
     if (G == 8 || G == 16 || G == 32)
         return (R == 0 || tile) && comp == mixed;
+    if (G == 128 || G == 256)
+        return (R == 0 || R == 8 || R == 32) && comp == mixed;
     return false;
+}
+
+// The register tile a workgroup mapping (G = 128, 256) uses for M points: the
+// smallest instantiated tile (8 or 32 points per lane) that holds them, or 0
+// (streamed) beyond.
+constexpr int workgroup_tile(uint32_t m, int g) {
+    const uint32_t per_lane = (m + uint32_t(g) - 1) / uint32_t(g);
+    return per_lane <= 8 ? 8 : per_lane <= 32 ? 32 : 0;
 }
 
 }  // namespace bfhip

@@ -2,7 +2,7 @@
 // spectra on CDNA4 (gfx950).
 //
 // Mapping: a group of G lanes owns one fit (G = 1 lane-per-fit, or 4/8/16/32/64
-// lanes; 4-lane groups only stream). With R > 0 the group holds its fit's points in registers, lane l of
+// lanes of a wave, or a workgroup of 128/256 lanes; 4-lane groups only stream). With R > 0 the group holds its fit's points in registers, lane l of
 // the group owning points l, l+G, …, l+G(R-1); y is read from HBM once per fit.
 // With R = 0 every pass streams x and y from memory (lane-per-fit form).
 //
@@ -1102,6 +1102,29 @@ __global__ void __launch_bounds__(256) lm_cov_kernel(FitArgs a) {
     Points<G, 0, stream_chunk(MODEL)> pts;
     pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
     covariance<MODEL, G, 0, S>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit]);
+}
+
+// Workgroup per fit (G = 128 or 256 lanes, 2 or 4 waves): every workgroup
+// solves its fits one after another (a grid-stride loop over the batch, so
+// the grid never exceeds the launch limits), with a register tile (x staged
+// once in LDS) or streamed points. All lanes of the workgroup follow the
+// same control path, so the barriers of the cross-wave reductions are
+// reached together.
+template <int MODEL, int G, int R, bool COMP, bool MIXED>
+__global__ void __launch_bounds__(G) lm_block_kernel(FitArgs a) {
+    static_assert(G == 128 || G == 256, "workgroup mappings are 2 or 4 waves");
+    using S = typename std::conditional<MIXED, double, float>::type;
+    if constexpr (R > 0) {
+        __shared__ float xs[G * R];
+        for (int j = threadIdx.x; j < G * R; j += G)
+            xs[j] = j < int(a.m) ? a.x[j] : 0.0f;
+        __syncthreads();
+        for (uint32_t fit = blockIdx.x; fit < a.n; fit += gridDim.x)
+            solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x), xs);
+    } else {
+        for (uint32_t fit = blockIdx.x; fit < a.n; fit += gridDim.x)
+            solve_fit<MODEL, G, R, COMP, S>(a, fit, int(threadIdx.x), a.x);
+    }
 }
 
 template <int MODEL, int G, int R, bool COMP, bool MIXED>

@@ -10,7 +10,9 @@ namespace bfhip {
     template <> hipError_t launch_lm_group<M, 4>(const Plan &, const FitArgs &, hipStream_t);  \
     template <> hipError_t launch_lm_group<M, 16>(const Plan &, const FitArgs &, hipStream_t); \
     template <> hipError_t launch_lm_group<M, 32>(const Plan &, const FitArgs &, hipStream_t); \
-    template <> hipError_t launch_lm_group<M, 64>(const Plan &, const FitArgs &, hipStream_t);
+    template <> hipError_t launch_lm_group<M, 64>(const Plan &, const FitArgs &, hipStream_t); \
+    template <> hipError_t launch_lm_group<M, 128>(const Plan &, const FitArgs &, hipStream_t); \
+    template <> hipError_t launch_lm_group<M, 256>(const Plan &, const FitArgs &, hipStream_t);
 BF_DECLARE_GROUPS(0)
 BF_DECLARE_GROUPS(1)
 BF_DECLARE_GROUPS(2)
@@ -33,6 +35,10 @@ BF_DECLARE_GROUPS(3)
             return launch_lm_group<M, 32>(plan, args, stream);                                 \
         case 64:                                                                               \
             return launch_lm_group<M, 64>(plan, args, stream);                                 \
+        case 128:                                                                              \
+            return launch_lm_group<M, 128>(plan, args, stream);                                \
+        case 256:                                                                              \
+            return launch_lm_group<M, 256>(plan, args, stream);                                \
         }                                                                                      \
         return hipErrorInvalidValue;                                                           \
     }                                                                                          \

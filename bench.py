@@ -22,6 +22,8 @@ of amplitude, seed 7, p0=None, bounds [0,-10,1e-4,0]–[inf,10,10,1]
                from an NPY file through the server (PCIe-inclusive);
 * doublets   — two_lorentzian N=262,144 M=2048 with uncertainties and 5 %
                degenerate rows (seed 11);
+* latency    — small batches (32-2048 fits, M 1024-8192): the throughput
+               table against the latency mapping (workgroups per fit);
 * cpu_baseline — the reference's own CPU path on a bounded sample.
 
 ``--only`` picks sections (production is always run unless
@@ -574,6 +576,62 @@ def bench_doublets(S, args):
 
 
 # --------------------------------------------------------------------------
+# Small batches: the throughput table against the latency mapping.
+# --------------------------------------------------------------------------
+LATENCY_CASES = [("pseudo_voigt", 1024, 32), ("pseudo_voigt", 4096, 256),
+                 ("pseudo_voigt", 8192, 2048), ("two_lorentzian", 2048, 256),
+                 ("two_lorentzian", 4096, 2048)]
+
+
+def bench_latency(S, args):
+    """LM time of small batches (device guess first, data generated in HBM
+    with the ragged workload's parameters, seed 41): mapping "auto" (the
+    throughput table by model and M) against "latency" (64/128/256 lanes per
+    fit when the batch cannot fill the device)."""
+    import batchfit_amd
+    from batchfit_amd.ragged import DeviceArray
+    lib, native = S.lib, S.native
+    lib.bf_hip_set_timing(1)
+    rows = []
+    for model, m, n in LATENCY_CASES:
+        dx = DeviceArray.of(np.linspace(-10, 10, m).astype(np.float32))
+        dy = DeviceArray(n * m * 4)
+        lo, hi = PARAM_RANGES[model]
+        noise, rel = NOISE[model]
+        S.synthesize(model, n, dx.ptr, m, lo, hi, dy.ptr, noise, rel, 41, 0,
+                     layout=1 if model == "two_lorentzian" else 0)
+        outs = [DeviceArray(n * 32), DeviceArray(n * 4), DeviceArray(n * 4), DeviceArray(n * 4)]
+        cfg = S.config(model, n, m, batchfit_amd.NUMERICS_DEFAULT)
+        row = {"model": model, "points": m, "fits": n}
+        for name, mapping in (("auto", native.MAP_AUTO), ("latency", native.MAP_LATENCY)):
+            launch = native.Launch()
+            launch.mapping = mapping
+            lm, call = [], []
+            for _ in range(5):
+                t = time.perf_counter()
+                S.check(lib.bf_hip_fit_device(native.MODELS[model], ctypes.byref(cfg),
+                                              ctypes.byref(launch), dx.ptr, dy.ptr, None,
+                                              *[o.ptr for o in outs], None, None, None, S.err,
+                                              len(S.err)))
+                times = (ctypes.c_double * 4)()
+                lib.bf_hip_last_timing(times, 4)
+                call.append((time.perf_counter() - t) * 1e3)
+                lm.append(times[3])
+            lanes, tile = batchfit_amd.plan(model, m, mapping=name, fits=n)
+            row[name] = {"lanes": lanes, "tile": tile, "lm_ms": float(np.median(lm[1:])),
+                         "call_ms": float(np.median(call[1:]))}
+        log(f"latency {model} M={m} N={n}: auto {row['auto']['lm_ms']:.3f} ms "
+            f"({row['auto']['lanes']} lanes) vs latency {row['latency']['lm_ms']:.3f} ms "
+            f"({row['latency']['lanes']} lanes)")
+        rows.append(row)
+        for a in outs + [dx, dy]:
+            a.free()
+    lib.bf_hip_set_timing(0)
+    return {"what": "median LM kernel time (HIP events) of 4 calls after one warm-up, from "
+                    "the device guess, numerics 5; seed 41", "cases": rows}
+
+
+# --------------------------------------------------------------------------
 def read_traffic():
     """HBM bytes per LM launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "lm_pseudo_voigt_m1024_counters.md")
@@ -626,7 +684,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=1 << 20)
-    ap.add_argument("--only", default="plain,ragged,serve,cube,doublets,cpu")
+    ap.add_argument("--only", default="plain,ragged,serve,cube,doublets,latency,cpu")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
     ap.add_argument("--ragged-fits", type=int, default=2_000_000)
@@ -681,6 +739,8 @@ def main():
         result["serving"] = bench_serving(S, args)
     if "doublets" in args.sections:
         result["doublets"] = bench_doublets(S, args)
+    if "latency" in args.sections:
+        result["small_batches"] = bench_latency(S, args)
     result["cpu_baseline"] = (cpu_baseline(x, y[: args.cpu_rows], bounds)
                               if "cpu" in args.sections else None)
     print(json.dumps(result), flush=True)

@@ -68,12 +68,18 @@ typedef struct bf_config {
 enum bf_mapping {
     BF_MAP_AUTO = 0,  /* tuned table by (model, M) */
     BF_MAP_LANE = 1,  /* one lane per fit, y streamed (point-major when transposed) */
-    BF_MAP_GROUP = 2  /* group_lanes lanes per fit, y in registers or streamed (see streamed) */
+    BF_MAP_GROUP = 2, /* group_lanes lanes per fit, y in registers or streamed (see streamed) */
+    BF_MAP_LATENCY = 3 /* by (model, M, N): a batch too small to fill the device in the
+                          tuned mapping takes 64, 128 or 256 lanes per fit (shorter latency;
+                          results then are those of that mapping), else as BF_MAP_AUTO */
 };
 
 typedef struct bf_launch {
     uint32_t mapping;     /* bf_mapping */
-    uint32_t group_lanes; /* 4, 8, 16, 32 or 64 for BF_MAP_GROUP (4 always streams); 0 = tuned */
+    uint32_t group_lanes; /* 4, 8, 16, 32 or 64 lanes of a wave for BF_MAP_GROUP (4 always
+                             streams), or 128 / 256 for a workgroup (2 or 4 waves) per fit
+                             (src/metal/cooperative.metal:16-32, KernelVariant::ThreadgroupPerFit,
+                             include/batchfit/metal/batch.hpp:19); 0 = tuned */
     uint32_t point_major; /* BF_MAP_LANE only: 1 = transpose y on the device first */
     uint32_t streamed;    /* BF_MAP_GROUP only: 1 = read the points from memory on every
                              pass instead of holding them in registers (same results) */
@@ -296,6 +302,11 @@ int bf_hip_model_eval(uint32_t model, uint32_t n, const float *x, const float *p
  * reference's kernel choice is internal (include/batchfit/metal/batch.hpp:21-50). */
 int bf_hip_plan(uint32_t model, uint32_t points, uint32_t numerics, const bf_launch *launch,
                 uint32_t *group_lanes, uint32_t *tile_points, char *err, size_t errlen);
+/* bf_hip_plan for a batch of `fits` spectra (BF_MAP_LATENCY depends on it;
+ * bf_hip_plan is this with fits = 0). */
+int bf_hip_plan_batch(uint32_t model, uint32_t points, uint32_t fits, uint32_t numerics,
+                      const bf_launch *launch, uint32_t *group_lanes, uint32_t *tile_points,
+                      char *err, size_t errlen);
 
 /* Enable (1) or disable (0) kernel timing with HIP events; also enabled by the
  * environment variable BF_HIP_TIMING. */

@@ -142,6 +142,19 @@ def test_plan_follows_the_tuned_table_without_a_device():
     assert plan("pseudo_voigt", 1024, mapping="group64_streamed") == (64, 0)
     assert plan("gaussian", 1500, mapping="group16") == (16, 0)
     assert plan("lorentzian", 100, mapping="lane") == (1, 0)
+    # Workgroups of 2 or 4 waves: tiles of 8 or 32 points per lane, else streamed.
+    assert plan("pseudo_voigt", 1024, mapping="group256") == (256, 8)
+    assert plan("pseudo_voigt", 4096, mapping="workgroup") == (256, 32)
+    assert plan("pseudo_voigt", 9000, mapping="group256") == (256, 0)
+    assert plan("two_lorentzian", 2048, mapping="group128") == (128, 32)
+    assert plan("gaussian", 100, mapping="group128_streamed") == (128, 0)
+    # The latency mapping: small batches take wide groups, large ones the table.
+    assert plan("pseudo_voigt", 1024, mapping="latency", fits=256) == (64, 16)
+    assert plan("pseudo_voigt", 4096, mapping="latency", fits=256) == (128, 32)
+    assert plan("pseudo_voigt", 8192, mapping="latency", fits=32) == (256, 32)
+    assert plan("pseudo_voigt", 20000, mapping="latency", fits=32) == (256, 0)
+    assert plan("pseudo_voigt", 1024, mapping="latency", fits=1 << 20) == plan("pseudo_voigt", 1024)
+    assert plan("pseudo_voigt", 4096, mapping="latency", fits=8192) == plan("pseudo_voigt", 4096)
     with pytest.raises(ValueError, match="numerics"):
         plan("gaussian", 256, numerics=1, mapping="group16")
 

@@ -18,7 +18,7 @@ import datasets
 
 pytestmark = pytest.mark.gpu
 
-MAPPINGS = ["lane", "group4_streamed", "group8", "group16", "group32", "group64"]
+MAPPINGS = ["lane", "group4_streamed", "group8", "group16", "group32", "group64", "group256"]
 NUMERICS = [0, 5]  # FP32 as the reference GPU kernels; compensated FP32 + FP64 solve
 P = {"gaussian": 3, "lorentzian": 3, "pseudo_voigt": 4, "two_lorentzian": 8}
 
@@ -27,7 +27,8 @@ def mappings_for(m):
     """Mappings whose register tile (at most 32 points per lane) holds M, and
     the 4-lane groups, which always stream."""
     return [mp for mp in MAPPINGS
-            if mp in ("lane", "group4_streamed", "group64") or (mp == "group32" and m <= 1024)
+            if mp in ("lane", "group4_streamed", "group64", "group256")
+            or (mp == "group32" and m <= 1024)
             or (mp == "group16" and m <= 512) or (mp == "group8" and m <= 256)]
 
 
@@ -344,7 +345,7 @@ def test_pathology_matrix(hip, golden, model):
     y = g[f"path_{model}_y"]
     for mapping in MAPPINGS:
         for numerics in (0, 1, 2, 3, 5, 7):
-            if mapping in ("group4_streamed", "group8", "group16", "group32") and \
+            if mapping in ("group4_streamed", "group8", "group16", "group32", "group256") and \
                     numerics not in (0, 5):
                 continue
             r = hip.fit(x, y, model, inits, None, "hip", mapping=mapping, numerics=numerics)
@@ -441,3 +442,50 @@ def test_point_count_edges(hip, oracle_port, m):
     for mapping in mappings_for(m):
         assert_fp32_gate(hip.fit(x, y, "pseudo_voigt", p0, bounds, "hip", mapping=mapping), ref,
                          f"M={m} {mapping}")
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("g,m", [(128, 129), (128, 1500), (256, 1024), (256, 2048), (256, 4096),
+                                 (256, 9000)])
+def test_workgroup_per_fit(hip, oracle_port, model, g, m):
+    """A workgroup of 2 or 4 waves per fit (the reference's threadgroup-per-fit
+    form, src/metal/cooperative.metal:16-32, 338-349): partials reduced within
+    each wave, then across the waves through LDS. Within the FP32 gate of the
+    oracle in both numerics, the register tile (8 or 32 points per lane) gives
+    the streamed form's bytes, and uncertainties leave the fit's bytes alone.
+    (Its sums run in another order than a 64-lane group's, so its bytes are
+    not a 64-lane group's.)"""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 33, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    ref = oracle_port.fit(x, y, model, p0, bounds, threads=8)
+    for numerics in NUMERICS:
+        a = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}", numerics=numerics)
+        assert_fp32_gate(a, ref, f"{model} M={m} group{g} numerics={numerics}")
+        b = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}_streamed",
+                    numerics=numerics)
+        fields_bitwise_equal(a, b, f"{model} M={m} group{g} tile vs streamed")
+    full = hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}", uncertainties=True)
+    fields_bitwise_equal(hip.FitResult(*full[:4]),
+                         hip.fit(x, y, model, p0, bounds, "hip", mapping=f"group{g}"),
+                         f"{model} M={m} group{g} uncertainties")
+    np.testing.assert_array_equal(full.covariance, full.covariance.transpose(0, 2, 1))
+    guessed = hip.fit(x, y, model, None, bounds, "hip", mapping=f"group{g}")
+    assert_fp32_gate(guessed, oracle_port.fit(x, y, model, None, bounds, threads=8),
+                     f"{model} M={m} group{g} from the device guess")
+
+
+@pytest.mark.parametrize("model,m,n", [("pseudo_voigt", 4096, 64), ("two_lorentzian", 2048, 48),
+                                       ("gaussian", 300, 2000)])
+def test_latency_mapping_is_the_mapping_it_names(hip, oracle_port, model, m, n):
+    """mapping="latency" runs the mapping plan(..., fits=N) names — the same
+    bytes as asking for it — within the FP32 gate."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    lanes, tile = hip.plan(model, m, mapping="latency", fits=n)
+    assert lanes >= 64
+    name = f"group{lanes}" + ("" if tile else "_streamed")
+    a = hip.fit(x, y, model, None, bounds, "hip", mapping="latency")
+    b = hip.fit(x, y, model, None, bounds, "hip", mapping=name)
+    fields_bitwise_equal(a, b, f"{model} M={m} N={n} latency = {name}")
+    assert_fp32_gate(a, oracle_port.fit(x, y, model, None, bounds, threads=8),
+                     f"{model} M={m} N={n} latency")
