@@ -1,6 +1,7 @@
 // bf_models.hip — per-model dispatch over the (model, group) launchers, the
 // guess kernels and the model-evaluation test kernels.
 #include "bf_launch.hpp"
+#include "bf_guess_group.hpp"
 
 namespace bfhip {
 
@@ -44,6 +45,9 @@ BF_DECLARE_GROUPS(3)
     }                                                                                          \
     template <> hipError_t launch_guess_model<M>(const GuessArgs &args, size_t lds_bytes,      \
                                                  hipStream_t stream) {                         \
+        hipError_t e = hipSuccess;                                                             \
+        if (launch_guess_group<M>(args, stream, e))                                            \
+            return e;                                                                          \
         hipLaunchKernelGGL((guess_kernel<M>), dim3(args.n), dim3(64), lds_bytes, stream, args); \
         return hipGetLastError();                                                              \
     }                                                                                          \
