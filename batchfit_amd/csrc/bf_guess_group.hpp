@@ -86,11 +86,68 @@ struct MedianKeys {
     uint32_t lower, upper;
 };
 
-// Median of the n valid keys of a group (initial_guess.hpp:25-30: the middle
+// A group's keys held in registers: S slots per lane, 0xffffffff in slots
+// that hold no value (above every finite key).
+template <int G, int S> struct RegKeys {
+    const uint32_t (&key)[S];
+    __device__ int count_le(uint32_t p) const { return group_count_le<G, S>(key, p); }
+    __device__ void range(uint32_t &kmin, uint32_t &kmax) const {
+        kmin = 0xffffffffu;
+        kmax = 0u;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            kmin = min(kmin, key[q]);
+            kmax = max(kmax, key[q] != 0xffffffffu ? key[q] : 0u);
+        }
+        kmin = group_min_u<G>(kmin);
+        kmax = group_max_u<G>(kmax);
+    }
+    // The largest key in [lo, hi] (0 if none).
+    __device__ uint32_t max_in(uint32_t lo, uint32_t hi) const {
+        uint32_t found = 0u;
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+            found = (key[q] >= lo && key[q] <= hi && key[q] > found) ? key[q] : found;
+        return group_max_u<G>(found);
+    }
+};
+
+// Keys read from memory at every use by one whole wave: key_at(e) for the
+// window's elements e < n (lane l takes l, l + 64, …).
+template <typename KeyAt> struct StreamKeys {
+    KeyAt key_at;
+    int n;
+    __device__ int count_le(uint32_t p) const {
+        int c = 0;
+        for (int e = int(threadIdx.x & 63); e < n; e += 64)
+            c += key_at(e) <= p ? 1 : 0;
+        return group_sum_i<64>(c);
+    }
+    __device__ void range(uint32_t &kmin, uint32_t &kmax) const {
+        kmin = 0xffffffffu;
+        kmax = 0u;
+        for (int e = int(threadIdx.x & 63); e < n; e += 64) {
+            const uint32_t k = key_at(e);
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
+        kmin = group_min_u<64>(kmin);
+        kmax = group_max_u<64>(kmax);
+    }
+    __device__ uint32_t max_in(uint32_t lo, uint32_t hi) const {
+        uint32_t found = 0u;
+        for (int e = int(threadIdx.x & 63); e < n; e += 64) {
+            const uint32_t k = key_at(e);
+            found = (k >= lo && k <= hi && k > found) ? k : found;
+        }
+        return group_max_u<64>(found);
+    }
+};
+
+// Median of the n keys of a key set (initial_guess.hpp:25-30: the middle
 // order statistic, and for an even n the midpoint of it and the largest value
-// below it); slots that hold no value carry 0xffffffff, above every finite
-// key. The k-th key (k = n/2) is searched in [lo_key, hi_key], which must
-// hold it, with lo keys below lo_key and hi keys at or below hi_key. Each
+// below it). The k-th key (k = n/2) is searched in [lo_key, hi_key], which
+// must hold it, with lo keys below lo_key and hi keys at or below hi_key. Each
 // step counts the keys at or below a pivot in [lo_key, hi_key) and keeps the
 // side that holds the k-th key, so the interval shrinks every step; the
 // pivot is the midpoint of the two end values as floats (noise around one
@@ -98,9 +155,9 @@ struct MedianKeys {
 // then the midpoint of the keys, so the search always ends. It stops when the
 // interval holds one key (the k-th) or one value. Each group keeps its own
 // state; the wave loops until every group of it has its key.
-template <int G, int S>
-__device__ MedianKeys group_median_in(const uint32_t (&key)[S], int n, uint32_t lo_key,
-                                      uint32_t hi_key, int lo, int hi) {
+template <typename Keys>
+__device__ MedianKeys median_in(const Keys &keys, int n, uint32_t lo_key, uint32_t hi_key, int lo,
+                                int hi) {
     const int k = n / 2;
     int steps = 0;
     bool done = lo_key == hi_key || hi - lo == 1;
@@ -110,7 +167,7 @@ __device__ MedianKeys group_median_in(const uint32_t (&key)[S], int n, uint32_t 
             uint32_t pivot = float_key(0.5f * key_float(lo_key) + 0.5f * key_float(hi_key));
             if (steps++ >= 12 || pivot < lo_key || pivot >= hi_key)
                 pivot = lo_key + (width >> 1);  // in [lo_key, hi_key)
-            const int c = group_count_le<G, S>(key, pivot);
+            const int c = keys.count_le(pivot);
             if (c < k + 1) {
                 lo_key = pivot + 1u;
                 lo = c;
@@ -122,54 +179,40 @@ __device__ MedianKeys group_median_in(const uint32_t (&key)[S], int n, uint32_t 
         }
     }
     // One key in [lo_key, hi_key] (or all of them one value): the k-th.
-    uint32_t t = lo_key;
-    if (lo_key != hi_key) {
-        uint32_t found = 0u;
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-            found = (key[q] >= lo_key && key[q] <= hi_key && key[q] > found) ? key[q] : found;
-        t = group_max_u<G>(found);
-    }
+    const uint32_t t = lo_key != hi_key ? keys.max_in(lo_key, hi_key) : lo_key;
     if (n & 1)
         return {key_float(t), t, t};
     // lo counts the keys below lo_key, and no key lies in [lo_key, t): so lo
-    // keys are below t.
-    uint32_t lower = t;
-    if (lo >= k) {  // the k smallest are all below t: the largest of them
-        uint32_t under = 0u;
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-            under = (key[q] < t && key[q] > under) ? key[q] : under;
-        lower = group_max_u<G>(under);
-    }
+    // keys are below t. If the k smallest are all below t, the lower middle
+    // is the largest of them; else it equals t.
+    const uint32_t lower = (lo >= k && t > 0u) ? keys.max_in(0u, t - 1u) : t;
     return {std_midpoint(key_float(lower), key_float(t)), lower, t};
 }
 
-// The search of group_median_in started on [a, b] (a ≤ b keys): if the k-th key
+// The search of median_in started on [a, b] (a ≤ b keys): if the k-th key
 // lies there, the search starts there; else on the side of it that holds the
 // k-th key. Two counts; any a ≤ b gives the same result.
-template <int G, int S>
-__device__ MedianKeys group_median_around(const uint32_t (&key)[S], int n, uint32_t a, uint32_t b) {
-    uint32_t kmin = 0xffffffffu, kmax = 0u;
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-        kmin = min(kmin, key[q]);
-        kmax = max(kmax, key[q] != 0xffffffffu ? key[q] : 0u);
-    }
-    kmin = group_min_u<G>(kmin);
-    kmax = group_max_u<G>(kmax);
+template <typename Keys>
+__device__ MedianKeys median_around(const Keys &keys, int n, uint32_t a, uint32_t b) {
+    uint32_t kmin, kmax;
+    keys.range(kmin, kmax);
     a = a < kmin ? kmin : a;
     b = b > kmax ? kmax : b;
     const int k = n / 2;
     if (a > b)  // an empty bracket: search everything
-        return group_median_in<G, S>(key, n, kmin, kmax, 0, n);
-    const int below_a = a > kmin ? group_count_le<G, S>(key, a - 1u) : 0;
-    const int upto_b = group_count_le<G, S>(key, b);
+        return median_in(keys, n, kmin, kmax, 0, n);
+    const int below_a = a > kmin ? keys.count_le(a - 1u) : 0;
+    const int upto_b = keys.count_le(b);
     if (below_a > k)  // the k-th key is below a
-        return group_median_in<G, S>(key, n, kmin, a - 1u, 0, below_a);
+        return median_in(keys, n, kmin, a - 1u, 0, below_a);
     if (upto_b <= k)  // the k-th key is above b
-        return group_median_in<G, S>(key, n, b + 1u, kmax, upto_b, n);
-    return group_median_in<G, S>(key, n, a, b, below_a, upto_b);
+        return median_in(keys, n, b + 1u, kmax, upto_b, n);
+    return median_in(keys, n, a, b, below_a, upto_b);
+}
+
+template <int G, int S>
+__device__ MedianKeys group_median_around(const uint32_t (&key)[S], int n, uint32_t a, uint32_t b) {
+    return median_around(RegKeys<G, S>{key}, n, a, b);
 }
 
 // Keys of mean ∓ σ/2 of the group's values (the floats of its valid keys):
@@ -287,6 +330,66 @@ __global__ void __launch_bounds__(256) guess_levels_kernel(GuessArgs a, int para
     const float spread = group_median_around<G, 2 * CE>(ku, 2 * count, ba, bb).value;
     if (live && gl == 0) {
         a.params[size_t(fit) * param_stride] = level;
+        a.params[size_t(fit) * param_stride + 1] = 1.4826f * spread;
+    }
+}
+
+// Stage 1 for edge windows beyond 1024 values (M ≥ 10,250): one wave per fit,
+// the same searches with the window's keys read from memory at every step
+// (the two edges stay in the L2 cache), so M has no limit.
+__global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a, int param_stride) {
+    const uint32_t fit = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = int(threadIdx.x & 63);
+    const bool live = fit < a.n;
+    const uint32_t f = live ? fit : a.n - 1;
+    const int m = int(a.m);
+    const int count = a.grid->count;
+    const float *y = a.y + size_t(f) * size_t(a.ys_fit);
+    const float *right = y + (m - count);
+    // mean ∓ σ/2 of a window, as mean_bracket.
+    auto bracket = [&](auto value_at, int n, uint32_t &lo, uint32_t &hi) {
+        float sum = 0.0f, sq = 0.0f;
+        for (int e = lane; e < n; e += 64) {
+            const float v = value_at(e);
+            sum += v;
+            sq += v * v;
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+            sum += __shfl_xor(sum, o, 64);
+            sq += __shfl_xor(sq, o, 64);
+        }
+        const float mean = sum / float(n), var = sq / float(n) - mean * mean;
+        const float half_sd = 0.5f * sqrtf(var > 0.0f ? var : 0.0f);
+        lo = float_key(mean - half_sd);
+        hi = float_key(mean + half_sd);
+        if (!(lo <= hi)) {
+            lo = 0u;
+            hi = 0xffffffffu;
+        }
+    };
+    auto left_v = [&](int e) { return y[e]; };
+    auto right_v = [&](int e) { return right[e]; };
+    auto union_v = [&](int e) { return e < count ? y[e] : right[e - count]; };
+    uint32_t ba, bb;
+    bracket(left_v, count, ba, bb);
+    auto lk = [&](int e) { return float_key(y[e]); };
+    const MedianKeys lm = median_around(StreamKeys<decltype(lk)>{lk, count}, count, ba, bb);
+    bracket(right_v, count, ba, bb);
+    auto rk = [&](int e) { return float_key(right[e]); };
+    const MedianKeys rm = median_around(StreamKeys<decltype(rk)>{rk, count}, count, ba, bb);
+    auto uk = [&](int e) { return float_key(union_v(e)); };
+    const MedianKeys um = median_around(StreamKeys<decltype(uk)>{uk, 2 * count}, 2 * count,
+                                        min(lm.lower, rm.lower), max(lm.upper, rm.upper));
+    const float left_level = lm.value, right_level = rm.value;
+    auto dev_v = [&](int e) {
+        return e < count ? fabsf(y[e] - left_level) : fabsf(right[e - count] - right_level);
+    };
+    bracket(dev_v, 2 * count, ba, bb);
+    auto dk = [&](int e) { return float_key(dev_v(e)); };
+    const float spread =
+        median_around(StreamKeys<decltype(dk)>{dk, 2 * count}, 2 * count, ba, bb).value;
+    if (live && lane == 0) {
+        a.params[size_t(fit) * param_stride] = um.value;
         a.params[size_t(fit) * param_stride + 1] = 1.4826f * spread;
     }
 }
@@ -576,12 +679,6 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     write();
 }
 
-// Edge values per lane of stage 1 for a window of `count` points: groups of 16
-// lanes up to 112 values, 64 lanes up to 1024; 0 beyond (the wave kernel).
-__host__ __device__ constexpr int guess_levels_lanes(uint32_t count) {
-    return count <= 112 ? 16 : count <= 1024 ? 64 : 0;
-}
-
 template <int G, int CE> hipError_t launch_guess_levels(const GuessArgs &a, int stride,
                                                         hipStream_t stream) {
     const uint32_t per_block = 256 / G;
@@ -590,9 +687,14 @@ template <int G, int CE> hipError_t launch_guess_levels(const GuessArgs &a, int 
     return hipGetLastError();
 }
 
-// Launches the two-stage guess when it serves (single-peak model, fit-major
-// y, edge windows of at most 1024 points: M ≤ 10249) and returns true; false
-// leaves the launch to the wave kernel.
+// Whether the two-stage guess serves a launch: single-peak model, fit-major y
+// (any M: stage 2 streams the row, stage 1 reads long edges from memory).
+__host__ __device__ constexpr bool guess_group_serves(int model, long long ys_pt) {
+    return model != TWO_LORENTZIAN && ys_pt == 1;
+}
+
+// Launches the two-stage guess when it serves and returns true; false leaves
+// the launch to the wave kernel.
 template <int MODEL>
 bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
     if constexpr (MODEL == TWO_LORENTZIAN) {
@@ -600,15 +702,19 @@ bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
     } else {
         constexpr int P = Params<MODEL>::P;
         const uint32_t count = a.m / 10 > 1 ? a.m / 10 : 1;
-        const int g = guess_levels_lanes(count);
-        if (a.ys_pt != 1 || a.n == 0 || g == 0)
+        if (!guess_group_serves(MODEL, a.ys_pt) || a.n == 0)
             return false;
-        if (g == 16)
+        if (count <= 112)
             e = launch_guess_levels<16, 7>(a, P, stream);
         else if (count <= 448)
             e = launch_guess_levels<64, 7>(a, P, stream);
-        else
+        else if (count <= 1024)
             e = launch_guess_levels<64, 16>(a, P, stream);
+        else {
+            hipLaunchKernelGGL(guess_levels_stream_kernel, dim3((a.n + 3) / 4), dim3(256), 0,
+                               stream, a, P);
+            e = hipGetLastError();
+        }
         if (e != hipSuccess)
             return true;
         hipLaunchKernelGGL((guess_peak_kernel<MODEL>), dim3((a.n + 15) / 16), dim3(256), 0,

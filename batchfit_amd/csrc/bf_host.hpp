@@ -35,6 +35,7 @@ struct Runtime {
     int status = BF_UNAVAILABLE;
     std::string why = "no HIP device";
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};  // the host-array entry's chunk streams
     std::mutex host_calls;  // serialises the synchronous host-array entry (bindings.cpp:13)
 
     Runtime() {

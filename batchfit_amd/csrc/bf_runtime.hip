@@ -8,6 +8,7 @@
 // HBM for the group mappings instead of a host 16×16 transpose.
 #include "bf_host.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -313,9 +314,12 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     if (!init) {
         const uint32_t sort_n = guess_extra_floats(int(model), m);
         const size_t lds = size_t(m + sort_n) * sizeof(float);
-        if (lds > 65536)
+        // The single-peak models' guess streams the row (no limit on M); the
+        // two-peak guess holds the signal in 64 KiB of LDS.
+        if (model == BF_TWO_LORENTZIAN && lds > 65536)
             return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "the device initial guess supports M up to 12,288 points; pass p0");
+                        "the device initial guess of two_lorentzian supports M up to 12,288 "
+                        "points; pass p0");
         float *guess = nullptr;
         if (fixed)
             guess = reinterpret_cast<float *>(fixed + 512);  // every row written by the guess
@@ -495,50 +499,108 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         return fail(err, errlen, BF_INVALID_ARGUMENT, "x, y and every output must be non-null");
     Runtime &rt = Runtime::get();
     std::lock_guard<std::mutex> lock(rt.host_calls);
-    bf_launch own{};
-    if (launch)
-        own = *launch;
-    own.stream = rt.stream;
+    // The batch goes through the device in chunks of about 64 MiB of y on two
+    // streams: while the device fits chunk k, the host copies chunk k + 1 in
+    // on the other stream. A fit's result depends on its own row, model, M
+    // and mapping only, so chunking changes no result.
+    for (hipStream_t &side : rt.side)
+        if (!side)
+            BF_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t chunk = std::min(n, std::max<size_t>(256, ((size_t(64) << 20) / (m * 4)) & ~size_t(255)));
+    const size_t chunks = (n + chunk - 1) / chunk, slots = chunks > 1 ? 2 : 1;
     hipStream_t s = rt.stream;
     Scratch scratch(s);
-    float *dx, *dy, *dp0 = nullptr, *dparams, *dchi2, *dcov = nullptr, *dse = nullptr;
-    int32_t *dstatus, *dus = nullptr;
-    uint32_t *dit;
+    float *dx;
     BF_TRY(scratch.get(reinterpret_cast<void **>(&dx), m * 4), "hipMallocAsync(x)");
-    BF_TRY(scratch.get(reinterpret_cast<void **>(&dy), n * m * 4), "hipMallocAsync(y)");
-    if (p0)
-        BF_TRY(scratch.get(reinterpret_cast<void **>(&dp0), n * P * 4), "hipMallocAsync(p0)");
-    // One output block: params | chi2 | status | iterations | cov | stderr | ustatus.
-    const size_t out_bytes = n * (P * 4 + 4 + 4 + 4) + (unc ? n * (P * P * 4 + P * 4 + 4) : 0);
-    char *dout = nullptr;
-    BF_TRY(scratch.get(reinterpret_cast<void **>(&dout), out_bytes), "hipMallocAsync(outputs)");
-    dparams = reinterpret_cast<float *>(dout);
-    dchi2 = dparams + n * P;
-    dstatus = reinterpret_cast<int32_t *>(dchi2 + n);
-    dit = reinterpret_cast<uint32_t *>(dstatus + n);
-    if (unc) {
-        dcov = reinterpret_cast<float *>(dit + n);
-        dse = dcov + n * P * P;
-        dus = reinterpret_cast<int32_t *>(dse + n * P);
-    }
     BF_TRY(hipMemcpyAsync(dx, x, m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(x)");
-    BF_TRY(hipMemcpyAsync(dy, y, n * m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(y)");
-    if (p0)
-        BF_TRY(hipMemcpyAsync(dp0, p0, n * P * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(p0)");
-    const int rc = fit_device(model, cfg, &own, dx, dy, dp0, dparams, dstatus, dit, dchi2, dcov,
-                              dse, dus, err, errlen);
-    if (rc != BF_OK)
-        return rc;
-    BF_TRY(hipMemcpyAsync(params, dparams, n * P * 4, hipMemcpyDeviceToHost, s), "copy params");
-    BF_TRY(hipMemcpyAsync(chi2, dchi2, n * 4, hipMemcpyDeviceToHost, s), "copy chi2");
-    BF_TRY(hipMemcpyAsync(status, dstatus, n * 4, hipMemcpyDeviceToHost, s), "copy status");
-    BF_TRY(hipMemcpyAsync(iterations, dit, n * 4, hipMemcpyDeviceToHost, s), "copy iterations");
-    if (unc) {
-        BF_TRY(hipMemcpyAsync(cov, dcov, n * P * P * 4, hipMemcpyDeviceToHost, s), "copy cov");
-        BF_TRY(hipMemcpyAsync(stderr_, dse, n * P * 4, hipMemcpyDeviceToHost, s), "copy stderr");
-        BF_TRY(hipMemcpyAsync(ustatus, dus, n * 4, hipMemcpyDeviceToHost, s), "copy ustatus");
+    // Per slot: y | p0 | params | chi2 | status | iterations | cov | stderr | ustatus.
+    const size_t out_per_fit = (P * 4 + 4 + 4 + 4) + (unc ? (P * P * 4 + P * 4 + 4) : 0);
+    const size_t slot_bytes = chunk * (m * 4 + (p0 ? P * 4 : 0) + out_per_fit);
+    char *dslot[2] = {nullptr, nullptr};
+    for (size_t k = 0; k < slots; ++k)
+        BF_TRY(scratch.get(reinterpret_cast<void **>(&dslot[k]), slot_bytes),
+               "hipMallocAsync(chunk buffers)");
+    BF_TRY(hipStreamSynchronize(s), "chunk buffer allocation");  // usable on the side streams
+    // Whatever way this call returns, the side streams finish with the chunk
+    // buffers before `scratch` frees them on the library stream.
+    struct SideDone {
+        Runtime &rt;
+        size_t slots;
+        ~SideDone() {
+            for (size_t k = 0; k < slots; ++k)
+                (void)hipStreamSynchronize(rt.side[k]);
+        }
+    } side_done{rt, slots};
+    struct Slot {
+        float *y, *p0, *params, *chi2, *cov, *se;
+        int32_t *status, *us;
+        uint32_t *it;
+    };
+    auto carve = [&](char *base) {
+        Slot v{};
+        v.y = reinterpret_cast<float *>(base);
+        v.p0 = v.y + chunk * m;
+        v.params = v.p0 + (p0 ? chunk * P : 0);
+        v.chi2 = v.params + chunk * P;
+        v.status = reinterpret_cast<int32_t *>(v.chi2 + chunk);
+        v.it = reinterpret_cast<uint32_t *>(v.status + chunk);
+        v.cov = reinterpret_cast<float *>(v.it + chunk);
+        v.se = v.cov + (unc ? chunk * P * P : 0);
+        v.us = reinterpret_cast<int32_t *>(v.se + (unc ? chunk * P : 0));
+        return v;
+    };
+    // Copies chunk c's results out (host arrays); ordered after its fit on its stream.
+    auto drain = [&](size_t c) -> int {
+        const Slot v = carve(dslot[c % slots]);
+        hipStream_t st = rt.side[c % slots];
+        const size_t f0 = c * chunk, nc = std::min(chunk, n - f0);
+        BF_TRY(hipMemcpyAsync(params + f0 * P, v.params, nc * P * 4, hipMemcpyDeviceToHost, st),
+               "copy params");
+        BF_TRY(hipMemcpyAsync(chi2 + f0, v.chi2, nc * 4, hipMemcpyDeviceToHost, st), "copy chi2");
+        BF_TRY(hipMemcpyAsync(status + f0, v.status, nc * 4, hipMemcpyDeviceToHost, st),
+               "copy status");
+        BF_TRY(hipMemcpyAsync(iterations + f0, v.it, nc * 4, hipMemcpyDeviceToHost, st),
+               "copy iterations");
+        if (unc) {
+            BF_TRY(hipMemcpyAsync(cov + f0 * P * P, v.cov, nc * P * P * 4, hipMemcpyDeviceToHost,
+                                  st),
+                   "copy cov");
+            BF_TRY(hipMemcpyAsync(stderr_ + f0 * P, v.se, nc * P * 4, hipMemcpyDeviceToHost, st),
+                   "copy stderr");
+            BF_TRY(hipMemcpyAsync(ustatus + f0, v.us, nc * 4, hipMemcpyDeviceToHost, st),
+                   "copy ustatus");
+        }
+        return BF_OK;
+    };
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c >= slots)  // the slot's previous chunk leaves before this one enters
+            if (const int rc = drain(c - slots))
+                return rc;
+        const Slot v = carve(dslot[c % slots]);
+        hipStream_t st = rt.side[c % slots];
+        const size_t f0 = c * chunk, nc = std::min(chunk, n - f0);
+        BF_TRY(hipMemcpyAsync(v.y, y + f0 * m, nc * m * 4, hipMemcpyHostToDevice, st),
+               "hipMemcpyAsync(y)");
+        if (p0)
+            BF_TRY(hipMemcpyAsync(v.p0, p0 + f0 * P, nc * P * 4, hipMemcpyHostToDevice, st),
+                   "hipMemcpyAsync(p0)");
+        bf_config part = *cfg;
+        part.fits = uint32_t(nc);
+        bf_launch own{};
+        if (launch)
+            own = *launch;
+        own.stream = st;
+        const int rc = fit_device(model, &part, &own, dx, v.y, p0 ? v.p0 : nullptr, v.params,
+                                  v.status, v.it, v.chi2, unc ? v.cov : nullptr,
+                                  unc ? v.se : nullptr, unc ? v.us : nullptr, err, errlen);
+        if (rc != BF_OK)
+            return rc;
     }
-    BF_TRY(hipStreamSynchronize(s), "fit completion");
+    for (size_t c = chunks > slots ? chunks - slots : 0; c < chunks; ++c)
+        if (const int rc = drain(c))
+            return rc;
+    for (size_t k = 0; k < slots; ++k)
+        BF_TRY(hipStreamSynchronize(rt.side[k]), "fit completion");
     return BF_OK;
 }
 
@@ -560,9 +622,9 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     const size_t P = size_t(kParams[model]);
     const uint32_t sort_n = guess_extra_floats(int(model), m);
     const size_t lds = size_t(m + sort_n) * sizeof(float);
-    if (lds > 65536)
+    if (model == BF_TWO_LORENTZIAN && lds > 65536)
         return fail(err, errlen, BF_INVALID_ARGUMENT,
-                    "the device initial guess supports M up to 12,288 points");
+                    "the device initial guess of two_lorentzian supports M up to 12,288 points");
     Scratch scratch(s);
     float *dx, *dy, *dp, *db;
     int32_t *dg;

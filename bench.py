@@ -12,6 +12,8 @@ of amplitude, seed 7, p0=None, bounds [0,-10,1e-4,0]–[inf,10,10,1]
 
 * plain      — the same batch through the plain single-shape launch (FP32,
                lane per fit, point-major y), the reference GPU organisation;
+* host       — the same batch from NumPy arrays through batchfit_amd.fit
+               (PCIe-inclusive: chunked copies overlapped with the fit);
 * ragged     — 2,000,000 fits, 25 % per model, M log-uniform in [64,4096]
                (seed 23), generated in HBM: one captured ragged launch vs one
                plain launch per (model, M) shape and vs one tuned launch per shape;
@@ -193,6 +195,25 @@ def bench_metric(S, args):
                     "mean_iterations": float(iterations.mean())},
         "mapping": form,
     }
+    if "host" in args.sections:
+        # The drop-in host path: batchfit_amd.fit on NumPy arrays (y crosses
+        # PCIe in chunks, overlapped with the fit of the previous chunk).
+        batchfit_amd.fit(x, y[:4096], "pseudo_voigt", None, bounds, "hip")  # warm-up
+        times = []
+        for _ in range(max(1, min(3, args.steps))):
+            t = time.perf_counter()
+            host = batchfit_amd.fit(x, y, "pseudo_voigt", None, bounds, "hip")
+            times.append(time.perf_counter() - t)
+        h_value = n / float(np.median(times))
+        same = bool(np.array_equal(host.status, status))
+        log(f"host arrays (PCIe-inclusive): {h_value:,.0f} fits/s ({np.median(times) * 1e3:.1f} ms "
+            f"per call); statuses equal to the device-resident run: {same}")
+        out["host_arrays"] = {"value": h_value, "unit": "fits/s",
+                              "ms_per_call": float(np.median(times) * 1e3),
+                              "what": "batchfit_amd.fit on NumPy float32 x, y (4 GiB): host to "
+                                      "device in chunks of 64 MiB on two streams, device guess + "
+                                      "fit, results back; median of calls after a warm-up",
+                              "statuses_equal_device_resident": same}
     if "plain" in args.sections:
         plain = native.Launch()
         plain.mapping, plain.point_major = native.MAP_LANE, 1
@@ -207,7 +228,9 @@ def bench_metric(S, args):
                                       "transpose": float(p_kms[:, 2].mean()),
                                       "lm": float(p_kms[:, 3].mean())},
                         "production_over_plain": value / p_value}
-    out["_host"] = (x, y, bounds)
+    params = dparams.download(np.empty((n, 4), np.float32))
+    chi2 = dchi2.download(np.empty(n, np.float32))
+    out["_host"] = (x, y, bounds, params, status, iterations, chi2)
     for a in (dx, dy, dparams, dstatus, diters, dchi2):
         a.free()
     return out
@@ -643,27 +666,125 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(x, y, bounds):
-    """The reference's own CPU path (oracle/_ref: cpu::fit_batch over its
-    ThreadPool) on the first rows of the metric workload, float32, p0=None."""
+def host_cpu():
+    """Core count and CPU model of this host (os.cpu_count, /proc/cpuinfo, lscpu)."""
+    import subprocess
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info = {"nproc": os.cpu_count(), "usable_cores": usable_cores(), "model": model}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, value = line.partition(":")
+            if key.strip() in ("Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                               "CPU(s)", "Model name"):
+                info[key.strip()] = value.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def usable_cores():
+    """Cores this process may use: its CPU affinity, capped by a cgroup v2 CPU
+    quota when one is set (a container's share can be below what nproc shows)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            cores = min(cores, max(1, -(-int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, cores)
+
+
+def fp64_check(ref, gpu_params, gpu_status, gpu_chi2):
+    """The GPU rows against the FP64 reference on the same float32 inputs:
+    status agreement, and for rows converged in both the relative errors of
+    amplitude and FWHM, the centre error in FWHM units, the mixing error and
+    the χ² relative error (max and RMS)."""
+    agree = gpu_status == ref.status
+    both = (gpu_status == 0) & (ref.status == 0)
+    g, r = gpu_params[both].astype(np.float64), ref.params[both]
+    err = {"amplitude": np.abs(g[:, 0] - r[:, 0]) / np.abs(r[:, 0]),
+           "centre_over_fwhm": np.abs(g[:, 1] - r[:, 1]) / np.abs(r[:, 2]),
+           "fwhm": np.abs(g[:, 2] - r[:, 2]) / np.abs(r[:, 2]),
+           "mixing_abs": np.abs(g[:, 3] - r[:, 3]),
+           "chi2": np.abs(gpu_chi2[both] - ref.chi2[both]) / ref.chi2[both]}
+    limit = {"amplitude": 1e-4, "centre_over_fwhm": 1e-4, "fwhm": 1e-4, "chi2": 1e-5}
+    rows = np.nonzero(both)[0]
+    over = {k: rows[err[k] > t] for k, t in limit.items()}
+    return {"rows": int(len(gpu_status)), "status_agreement": float(agree.mean()),
+            "status_disagreements": int((~agree).sum()), "converged_in_both": int(both.sum()),
+            "max": {k: float(v.max()) if v.size else None for k, v in err.items()},
+            "rms": {k: float(np.sqrt(np.mean(v ** 2))) if v.size else None for k, v in err.items()},
+            "within_gate": {k: float(1 - over[k].size / max(1, both.sum())) for k in limit},
+            "rows_over_gate": {k: [int(i) for i in over[k][:8]] for k in limit},
+            "gate": "status >= 99.9%; amplitude, fwhm <= 1e-4 relative; centre <= 1e-4 fwhm; "
+                    "chi2 <= 1e-5 relative"}
+
+
+def cpu_baseline(x, y, bounds, gpu=None):
+    """The reference's own CPU path (oracle/_ref: cpu::initial_guess + fit_batch
+    over its ThreadPool) on the first rows of the metric workload, p0=None,
+    float32 and float64 (the float32 inputs promoted), with all host cores and
+    with one worker (a smaller sample). Also checks the GPU's rows against the
+    float64 results."""
     from oracle import oracle
     kind = "reference"
     try:
         lib = oracle.load("reference")
     except (FileNotFoundError, OSError):
         kind, lib = "port", oracle.load("port")
-    threads = min(16, os.cpu_count() or 1)
-    lib.fit(x, y[:256], "pseudo_voigt", None, bounds, threads=threads)  # warm-up
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        lib.fit(x, y, "pseudo_voigt", None, bounds, threads=threads)
-        times.append(time.perf_counter() - t)
-    value = y.shape[0] / float(np.median(times))
-    log(f"cpu baseline ({kind}, {threads} threads): {value:,.0f} fits/s")
-    return {"value": value, "unit": "fits/s", "cores": threads, "kind": kind,
-            "sample": f"first {y.shape[0]} rows of the metric workload, float32, p0=None "
-                      "(guess included), median of 3 after a 256-fit warm-up"}
+    cores = usable_cores()
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
+    one_rows = min(len(y), 4096)
+
+    def timed(xx, yy, threads):
+        lib.fit(xx, yy[:256], "pseudo_voigt", None, bounds, threads=threads)  # warm-up
+        times, res = [], None
+        for _ in range(3):
+            t = time.perf_counter()
+            res = lib.fit(xx, yy, "pseudo_voigt", None, bounds, threads=threads)
+            times.append(time.perf_counter() - t)
+        return yy.shape[0] / float(np.median(times)), res
+
+    legs = {}
+    f32_all, ref32 = timed(x, y, cores)
+    f64_all, ref64 = timed(x64, y64, cores)
+    f32_one, _ = timed(x, y[:one_rows], 1)
+    f64_one, _ = timed(x64, y64[:one_rows], 1)
+    legs = {"float32_all_cores": f32_all, "float64_all_cores": f64_all,
+            "float32_one_worker": f32_one, "float64_one_worker": f64_one}
+    log(f"cpu baseline ({kind}, {cores} threads): f32 {f32_all:,.0f} fits/s, f64 {f64_all:,.0f}; "
+        f"one worker f32 {f32_one:,.0f}, f64 {f64_one:,.0f}")
+    out = {"value": f32_all, "unit": "fits/s", "cores": cores, "kind": kind,
+           "sample": f"first {y.shape[0]} rows of the metric workload (all cores) and first "
+                     f"{one_rows} (one worker), p0=None (guess included), median of 3 calls "
+                     "after a 256-fit warm-up; float64 legs fit the same float32 values "
+                     "promoted",
+           "legs_fits_per_s": legs, "host": host_cpu()}
+    if gpu is not None:
+        params, status, chi2 = gpu
+        n = y.shape[0]
+        out["fp64_check"] = fp64_check(ref64, params[:n], status[:n], chi2[:n])
+        c = out["fp64_check"]
+        log(f"GPU vs FP64 on {n} rows: status agreement {c['status_agreement']:.5f}; max "
+            f"amplitude {c['max']['amplitude']:.3g}, fwhm {c['max']['fwhm']:.3g}, centre/fwhm "
+            f"{c['max']['centre_over_fwhm']:.3g}, chi2 {c['max']['chi2']:.3g}; within the gate "
+            + ", ".join(f"{k} {v:.5f}" for k, v in c["within_gate"].items()))
+        # The reference's own float32 path against its float64 path, the same way.
+        out["reference_float32_vs_fp64"] = fp64_check(ref64, ref32.params, ref32.status,
+                                                      ref32.chi2)["within_gate"]
+    return out
 
 
 def calibrate_copy():
@@ -684,7 +805,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=1 << 20)
-    ap.add_argument("--only", default="plain,ragged,serve,cube,doublets,latency,cpu")
+    ap.add_argument("--only", default="plain,host,ragged,serve,cube,doublets,latency,cpu")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
     ap.add_argument("--ragged-fits", type=int, default=2_000_000)
@@ -704,7 +825,7 @@ def main():
         args.sections.discard("cpu")
     S = Session()
     metric = bench_metric(S, args)
-    x, y, bounds = metric.pop("_host")
+    x, y, bounds, g_params, g_status, g_iters, g_chi2 = metric.pop("_host")
     result = {
         "metric": METRIC,
         "value": metric["value"],
@@ -733,6 +854,8 @@ def main():
     }
     if "plain" in metric:
         result["plain"] = metric["plain"]
+    if "host_arrays" in metric:
+        result["host_arrays"] = metric["host_arrays"]
     if "ragged" in args.sections:
         result["ragged"] = bench_ragged(S, args)
     if "serve" in args.sections:
@@ -741,8 +864,11 @@ def main():
         result["doublets"] = bench_doublets(S, args)
     if "latency" in args.sections:
         result["small_batches"] = bench_latency(S, args)
-    result["cpu_baseline"] = (cpu_baseline(x, y[: args.cpu_rows], bounds)
+    result["cpu_baseline"] = (cpu_baseline(x, y[: args.cpu_rows], bounds,
+                                           (g_params, g_status, g_chi2))
                               if "cpu" in args.sections else None)
+    if result["cpu_baseline"]:
+        result["fp64_check"] = result["cpu_baseline"].pop("fp64_check", None)
     print(json.dumps(result), flush=True)
 
 

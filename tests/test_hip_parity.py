@@ -489,3 +489,72 @@ def test_latency_mapping_is_the_mapping_it_names(hip, oracle_port, model, m, n):
     fields_bitwise_equal(a, b, f"{model} M={m} N={n} latency = {name}")
     assert_fp32_gate(a, oracle_port.fit(x, y, model, None, bounds, threads=8),
                      f"{model} M={m} N={n} latency")
+
+
+@pytest.mark.parametrize("unc,given", [(False, False), (True, True)])
+def test_host_arrays_stream_in_chunks(hip, unc, given):
+    """bf_hip_fit (host arrays) takes the batch through the device in chunks of
+    about 64 MiB of y on two streams (copy-in of one chunk beside the fit of
+    another): the same bytes as one device call on the whole batch."""
+    import ctypes
+    from batchfit_amd import _native
+    from batchfit_amd.ragged import DeviceArray
+    x, y, bounds, _ = datasets.config2(rows=40000)  # 3 chunks of 16,384 fits at M = 1024
+    n, m = y.shape
+    p0 = np.tile(np.array([2.0, 0.0, 1.0, 0.5], np.float32), (n, 1)) if given else None
+    host = hip.fit(x, y, "pseudo_voigt", p0, bounds, "hip", uncertainties=unc)
+    lib = _native.library()
+    cfg = _native.default_config()
+    cfg.fits, cfg.points, cfg.numerics = n, m, hip.NUMERICS_DEFAULT
+    for k in range(4):
+        cfg.lower[k], cfg.upper[k] = bounds[0][k], bounds[1][k]
+    dx, dy = DeviceArray.of(x), DeviceArray.of(y)
+    dp0 = DeviceArray.of(p0) if given else None
+    outs = [DeviceArray(n * 16), DeviceArray(n * 4), DeviceArray(n * 4), DeviceArray(n * 4)]
+    if unc:
+        outs += [DeviceArray(n * 64), DeviceArray(n * 16), DeviceArray(n * 4)]
+    ptrs = [o.ptr for o in outs] + ([] if unc else [None] * 3)
+    err = _native.error_buffer()
+    _native.check(lib.bf_hip_fit_device(2, ctypes.byref(cfg), None, dx.ptr, dy.ptr,
+                                        dp0.ptr if given else None, *ptrs, err, len(err)), err)
+    shapes = [(n, 4), (n,), (n,), (n,), (n, 4, 4), (n, 4), (n,)]
+    types = [np.float32, np.int32, np.uint32, np.float32, np.float32, np.float32, np.int32]
+    dev = [o.download(np.empty(sh, t)) for o, sh, t in zip(outs, shapes, types)]
+    order = [host.params, host.status, host.iterations, host.chi2] + (
+        [host.covariance, host.stderr, host.uncertainty_status] if unc else [])
+    for a, b in zip(order, dev):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8), b.view(np.uint8))
+    for o in outs + [dx, dy] + ([dp0] if given else []):
+        o.free()
+
+
+@pytest.mark.parametrize("model", ["gaussian", "lorentzian", "pseudo_voigt"])
+@pytest.mark.parametrize("m", [10250, 16384, 65536])
+def test_guess_long_spectra(hip, oracle_port, model, m):
+    """The single-peak guess has no limit on M: edge windows over 1024 values
+    are searched with their keys read from memory, and the row streams. Bit
+    for bit the oracle's guess (initial_guess.hpp keeps its scratch on the
+    heap, :143); fits from it within the FP32 gate."""
+    x, y, p0, bounds, _ = datasets.comparison_set(model, 8, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    got = device_guess(x, y, model)
+    ref = oracle_port.guess(x, y, model, threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8)), (model, m)
+    assert np.all(ref[2] == 0)
+    q = (np.round(y * 50) / 50).astype(np.float32)
+    got = device_guess(x, q, model)
+    ref = oracle_port.guess(x, q, model, threads=8)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
+    fit = hip.fit(x, y, model, None, bounds, "hip")
+    assert_fp32_gate(fit, oracle_port.fit(x, y, model, None, bounds, threads=8),
+                     f"{model} M={m} guessed")
+
+
+def test_two_peak_guess_length_limit(hip):
+    """The two-peak guess holds the signal in LDS: M up to 12,288 points."""
+    x = np.linspace(-5, 5, 12289).astype(np.float32)
+    with pytest.raises(ValueError, match="12,288"):
+        hip.fit(x, np.ones((2, x.size), np.float32), "two_lorentzian", None, None, "hip")
