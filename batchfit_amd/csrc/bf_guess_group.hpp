@@ -19,41 +19,38 @@
 
 namespace bfhip {
 
-template <int G> __device__ __forceinline__ uint32_t group_min_u(uint32_t v) {
-    v = min(v, uint32_t(xlane<1>(int(v))));
-    v = min(v, uint32_t(xlane<2>(int(v))));
-    v = min(v, uint32_t(xlane<4>(int(v))));
-    v = min(v, uint32_t(xlane<8>(int(v))));
+// Reductions of 32-bit integers over aligned groups of G = 2 … 64 lanes
+// (every lane gets the result): xlane<1>, <2> (quad_perm), <4> (row_half_mirror), <8> (row_mirror),
+// <16> (ds_swizzle), <32> (across the halves).
+template <int G, typename T, typename Op> __device__ __forceinline__ T group_fold(T v, Op op) {
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "G is a power of two up to 64");
+    static_assert(sizeof(T) == 4, "32-bit integers");
+    auto move = [](auto step, T w) { return T(xlane<decltype(step)::value>(int(w))); };
+    v = op(v, move(std::integral_constant<int, 1>{}, v));
+    if constexpr (G >= 4)
+        v = op(v, move(std::integral_constant<int, 2>{}, v));
+    if constexpr (G >= 8)
+        v = op(v, move(std::integral_constant<int, 4>{}, v));
+    if constexpr (G >= 16)
+        v = op(v, move(std::integral_constant<int, 8>{}, v));
     if constexpr (G >= 32)
-        v = min(v, uint32_t(xlane<16>(int(v))));
+        v = op(v, move(std::integral_constant<int, 16>{}, v));
     if constexpr (G >= 64)
-        v = min(v, uint32_t(xlane<32>(int(v))));
+        v = op(v, move(std::integral_constant<int, 32>{}, v));
     return v;
+}
+template <int G> __device__ __forceinline__ uint32_t group_min_u(uint32_t v) {
+    return group_fold<G>(v, [](uint32_t a, uint32_t b) { return min(a, b); });
 }
 template <int G> __device__ __forceinline__ uint32_t group_max_u(uint32_t v) {
-    v = max(v, uint32_t(xlane<1>(int(v))));
-    v = max(v, uint32_t(xlane<2>(int(v))));
-    v = max(v, uint32_t(xlane<4>(int(v))));
-    v = max(v, uint32_t(xlane<8>(int(v))));
-    if constexpr (G >= 32)
-        v = max(v, uint32_t(xlane<16>(int(v))));
-    if constexpr (G >= 64)
-        v = max(v, uint32_t(xlane<32>(int(v))));
-    return v;
+    return group_fold<G>(v, [](uint32_t a, uint32_t b) { return max(a, b); });
 }
 template <int G> __device__ __forceinline__ int group_sum_i(int v) {
-    v += xlane<1>(v);
-    v += xlane<2>(v);
-    v += xlane<4>(v);
-    v += xlane<8>(v);
-    if constexpr (G >= 32)
-        v += xlane<16>(v);
-    if constexpr (G >= 64)
-        v += xlane<32>(v);
-    return v;
+    return group_fold<G>(v, [](int a, int b) { return a + b; });
 }
 // Largest value of the group, ties to the smallest index (finite values).
 template <int G> __device__ __forceinline__ void group_argmax(float &v, int &idx) {
+    static_assert(G >= 16 && G <= 64, "groups of 16 to 64 lanes");
     auto step = [&](float w, int wi) {
         const bool take = w > v || (w == v && wi < idx);
         v = take ? w : v;
@@ -394,6 +391,203 @@ __global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a, i
     }
 }
 
+// Stage 1 for edge windows of up to 128 values (M < 1,290) by sorting: one
+// fit per group of 8 lanes (eight per wave), each edge's keys in 8·S slots,
+// element e = (lane in the group)·S + slot. The bitonic networks are in the
+// form whose comparators all put the smaller key at the lower position (each
+// merge of sorted halves opens with the mirror comparator e ↔ e ^ (2^p − 1)):
+// a pair inside a lane is a min and a max, a pair across lanes a DPP move
+// (quad_perm xor 1, reverse of a quad, xor 2, row_half_mirror) or a ds_swizzle
+// (xor 4) and a min or max. The edge medians are read at fixed positions. For
+// the medians of unions the half-cleaner identity serves: for X ascending and
+// Y descending, c values each, min(X[i], Y[i]) over i < c are the c smallest
+// of X ∪ Y and max(X[i], Y[i]) the c largest. No LDS and no data-dependent
+// loops.
+// Keys of two slots of one lane in order.
+__device__ __forceinline__ void key_cx(uint32_t &a, uint32_t &b) {
+    const uint32_t lo = min(a, b);
+    b = max(a, b);
+    a = lo;
+}
+// Move of a key from the partner lane: CTRL 1 (xor 1), 2 (xor 2), 3 (reverse
+// of the quad: xor 3), 4 (xor 4, ds_swizzle), 7 (row_half_mirror: xor 7).
+// Every source lane lies in the row (bound_ctrl never applies).
+template <int CTRL> __device__ __forceinline__ uint32_t key_from(uint32_t v) {
+    constexpr int ctrl = CTRL == 1 ? 0xB1 : CTRL == 2 ? 0x4E : CTRL == 3 ? 0x1B : 0x141;
+    if constexpr (CTRL == 4)
+        return uint32_t(__builtin_amdgcn_ds_swizzle(int(v), 0x101F));
+    else
+        return uint32_t(__builtin_amdgcn_mov_dpp(int(v), ctrl, 0xF, 0xF, true));
+}
+// Comparators e ↔ e ^ X inside each lane (X < S).
+template <int S, int X> __device__ __forceinline__ void lane_stage(uint32_t (&v)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (s < (s ^ X))
+            key_cx(v[s], v[s ^ X]);
+}
+template <int S, int X> __device__ __forceinline__ void lane_xors(uint32_t (&v)[S]) {
+    if constexpr (X >= 1) {
+        lane_stage<S, X>(v);
+        lane_xors<S, X / 2>(v);
+    }
+}
+// Merges of blocks of B = 2, 4, … S slots inside each lane.
+template <int S, int B> __device__ __forceinline__ void lane_merges(uint32_t (&v)[S]) {
+    if constexpr (B <= S) {
+        lane_stage<S, B - 1>(v);
+        lane_xors<S, B / 4>(v);
+        lane_merges<S, B * 2>(v);
+    }
+}
+// Comparators across lanes: the partner lane by CTRL, the same slot or (MIRROR)
+// the reversed slot; the lane whose bit is set keeps the larger key.
+template <int S, int CTRL, bool MIRROR>
+__device__ __forceinline__ void cross_stage(uint32_t (&v)[S], bool upper) {
+    uint32_t lo[S], hi[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t src = v[MIRROR ? S - 1 - s : s];
+        const uint32_t o = key_from<CTRL>(src);
+        lo[s] = min(o, v[s]);
+        hi[s] = max(o, v[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        v[s] = upper ? hi[s] : lo[s];
+}
+// Ascending sort of the group's 8·S keys.
+template <int S> __device__ __forceinline__ void sort_keys8(uint32_t (&v)[S], int gl) {
+    lane_merges<S, 2>(v);
+    cross_stage<S, 1, true>(v, gl & 1);  // blocks of 2 lanes
+    lane_xors<S, S / 2>(v);
+    cross_stage<S, 3, true>(v, gl & 2);  // blocks of 4 lanes
+    cross_stage<S, 1, false>(v, gl & 1);
+    lane_xors<S, S / 2>(v);
+    cross_stage<S, 7, true>(v, gl & 4);  // all 8 lanes
+    cross_stage<S, 2, false>(v, gl & 2);
+    cross_stage<S, 1, false>(v, gl & 1);
+    lane_xors<S, S / 2>(v);
+}
+// Ascending order of a bitonic sequence of the group's 8·S keys.
+template <int S> __device__ __forceinline__ void merge_keys8(uint32_t (&v)[S], int gl) {
+    cross_stage<S, 4, false>(v, gl & 4);
+    cross_stage<S, 2, false>(v, gl & 2);
+    cross_stage<S, 1, false>(v, gl & 1);
+    lane_xors<S, S / 2>(v);
+}
+// The key at position e (the same for every group of the wave).
+template <int S> __device__ __forceinline__ uint32_t key_at8(const uint32_t (&v)[S], int e, int gl) {
+    // Halving by the slot's bits, high to low (selects, not an indexed read:
+    // registers cannot be indexed per group).
+    const int slot = e & (S - 1);
+    uint32_t t[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        t[s] = v[s];
+#pragma unroll
+    for (int w = S / 2; w >= 1; w /= 2) {
+        const bool upper = (slot & w) != 0;
+#pragma unroll
+        for (int s = 0; s < w; ++s)
+            t[s] = upper ? t[s + w] : t[s];
+    }
+    return group_max_u<8>(gl == e / S ? t[0] : 0u);
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) guess_levels_sort_kernel(GuessArgs a, int param_stride) {
+    constexpr int G = 8, N = G * S;
+    const int gl = int(threadIdx.x & (G - 1));
+    const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool live = fit < a.n;
+    const uint32_t f = live ? fit : a.n - 1;
+    const int m = int(a.m);
+    const int c = a.grid->count;  // 1 … N
+    const float *y = a.y + size_t(f) * size_t(a.ys_fit);
+    const float *right = y + (m - c);
+    // Left edge ascending (pads above every key, at the end); right edge
+    // descending, held as complemented keys (pads: key 0, at the end). Which
+    // slot a value starts in does not matter: lane gl loads values gl, gl + 8, ….
+    // Loads at a clamped index (in the row; every load issued at once), the
+    // pads chosen after.
+    float lv[S], rv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = min(s * G + gl, c - 1);
+        lv[s] = y[i];
+        rv[s] = right[i];
+    }
+    uint32_t lk[S], rk[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const bool in = s * G + gl < c;
+        lk[s] = in ? float_key(lv[s]) : 0xffffffffu;
+        rk[s] = in ? ~float_key(rv[s]) : 0xffffffffu;
+    }
+    sort_keys8<S>(lk, gl);
+    sort_keys8<S>(rk, gl);
+    // Each edge's median (initial_guess.hpp:25-30): ascending positions k − 1
+    // and k, k = c / 2 (k alone for odd c); the right edge's ascending position
+    // t is descending position c − 1 − t.
+    const int k = c / 2;
+    const float left_hi = key_float(key_at8<S>(lk, k, gl));
+    const float right_hi = key_float(~key_at8<S>(rk, c - 1 - k, gl));
+    float left_level = left_hi, right_level = right_hi;
+    if (!(c & 1)) {
+        left_level = std_midpoint(key_float(key_at8<S>(lk, k - 1, gl)), left_hi);
+        right_level = std_midpoint(key_float(~key_at8<S>(rk, c - k, gl)), right_hi);
+    }
+    // Median of both edges: positions c − 1 and c of the 2c values.
+    uint32_t below = 0u, above = 0xffffffffu;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t r = ~rk[s];
+        below = max(below, min(lk[s], r));
+        above = min(above, max(lk[s], r));
+    }
+    below = group_max_u<G>(below);
+    above = group_min_u<G>(above);
+    const float level = std_midpoint(key_float(below), key_float(above));
+    // Absolute deviations from each edge's own median: along a sorted edge they
+    // fall to the median and rise after it, and the pads (above every key)
+    // follow, so each edge is a bitonic sequence, ordered by one merge. The
+    // right edge is ordered descending (complemented keys), so the two
+    // together (left ascending, then right descending) are bitonic again: the
+    // half-cleaner min(left[i], right[i]) holds the N smallest, merged once
+    // more; the pads are above every deviation, so positions c − 1 and c are
+    // the median pair of the 2c deviations.
+    uint32_t ld[S], rd[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = gl * S + s;
+        const float dl = fabsf(key_float(lk[s]) - left_level);
+        const float dr = fabsf(key_float(~rk[s]) - right_level);
+        ld[s] = e < c ? (__float_as_uint(dl) | 0x80000000u) : 0xffffffffu;
+        rd[s] = e < c ? ~(__float_as_uint(dr) | 0x80000000u) : 0u;
+    }
+    merge_keys8<S>(ld, gl);
+    merge_keys8<S>(rd, gl);
+    uint32_t top = 0xffffffffu;
+    if (c == N) {  // position c lies in the upper half
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            top = min(top, max(ld[s], ~rd[s]));
+        top = group_min_u<G>(top);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        ld[s] = min(ld[s], ~rd[s]);
+    merge_keys8<S>(ld, gl);
+    const uint32_t d_lo = key_at8<S>(ld, c - 1, gl);
+    const uint32_t d_hi = c == N ? top : key_at8<S>(ld, c, gl);
+    const float spread = std_midpoint(key_float(d_lo), key_float(d_hi));
+    if (live && gl == 0) {
+        a.params[size_t(fit) * param_stride] = level;
+        a.params[size_t(fit) * param_stride + 1] = 1.4826f * spread;
+    }
+}
+
 // Stage 2 of the guess, one fit per group of 16 lanes (four per wave): lane gl
 // takes the points gl, gl + 16, … as it streams the row in chunks of 16·CH
 // points (coalesced; a sliding window of slots, so the registers do not grow
@@ -407,7 +601,7 @@ __global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a, i
 template <int MODEL>
 __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     static_assert(MODEL != TWO_LORENTZIAN, "the two-peak guess runs one wave per fit");
-    constexpr int G = 16, CH = 8;
+    constexpr int G = 16;
     constexpr int P = Params<MODEL>::P;
     const int lane = int(threadIdx.x & 63);
     const int gl = lane & (G - 1);
@@ -448,99 +642,197 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     auto sig = [&](float yj, float xj) {
         return neg_zero ? yj - (level + 0.0f * (xj - grid.left_x)) : yj - level;
     };
-    const int slots = (m + G - 1) / G;
-    // Strongest smoothed local maximum (initial_guess.hpp:170-182), each lane
-    // in increasing index keeping its first best; the slot being tested lags
-    // the slot being smoothed by one, which lags the newest loaded by one.
-    int ybad = 0, sbad = 0;
+    // Strongest smoothed local maximum (initial_guess.hpp:170-182). The first
+    // largest smoothed value over 1 ≤ j ≤ M − 2 is that maximum (nothing
+    // before it is as large, nothing after it larger), unless it sits at j = 1
+    // or M − 2 and the row's end point beats it: then, and only then, the
+    // reference's scan is replayed point by point (replay_scan). The fast pass
+    // streams the row 64 points per step, four consecutive points per lane
+    // (one 16-byte load when the rows are 16-byte aligned), so a point's
+    // neighbours sit in its own lane except at the lane's two ends (DPP).
+    // Each lane keeps its largest smoothed value and the first step that
+    // reached it; the lanes holding the group's maximum recompute that step's
+    // four points to find its first index.
     float best = 0.0f;
     int first = 0x7fffffff;
-    float v_prev = 0.0f, v_cur = 0.0f, sm_prev = 0.0f, sm_cur = 0.0f;
-    // One chunk of CH slots from slot k0. INNER: every point the chunk loads,
-    // smooths or tests lies inside the row (1 ≤ j ≤ M − 2), so no lane needs
-    // a range test; NZ: the level is −0 (the grid is read).
-    auto chunk = [&](int k0, auto inner_tag, auto nz_tag) {
-        constexpr bool INNER = decltype(inner_tag)::value, NZ = decltype(nz_tag)::value;
-        float vin[CH];
+    int sbad = 0;
+    {
+        constexpr int CS = 4;  // steps per chunk (loads in flight)
+        const int steps = (m + 63) / 64;
+        float lane_best = -__builtin_inff();
+        int lane_step = -1;
+        float nan_acc = 0.0f;  // Σ s·0: NaN once a signal value is not finite
+        // Quarter and half of the four points of the step being smoothed, and
+        // the quarter of the last point of the step before it.
+        float pq3 = 0.0f, cq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ch[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        // FULL: the step lies inside the row; VEC: 16-byte rows (one load).
+        auto load4 = [&](int k, float(&s)[4], auto full_tag, auto vec_tag, auto nz_tag) {
+            constexpr bool FULL = decltype(full_tag)::value, VEC = decltype(vec_tag)::value;
+            constexpr bool NZ = decltype(nz_tag)::value;
+            const int p = 64 * k + 4 * gl;
+            if constexpr (VEC) {  // M is a multiple of 4: a load is all in or all out
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const bool in = FULL || p < m;
+                if (in)
+                    v = *reinterpret_cast<const float4 *>(y + p);
+                s[0] = in ? v.x - level : 0.0f;
+                s[1] = in ? v.y - level : 0.0f;
+                s[2] = in ? v.z - level : 0.0f;
+                s[3] = in ? v.w - level : 0.0f;
+            } else {
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int j = gl + G * (k0 + c);
-            float yj = 0.0f, xj = grid.left_x;
-            if (INNER || j < m) {
-                yj = y[j];
-                if (NZ)
-                    xj = x[j];
-            }
-            const float s = NZ ? yj - (level + 0.0f * (xj - grid.left_x)) : yj - level;
-            ybad |= !finite_f(yj);
-            sbad |= !finite_f(s);
-            vin[c] = (INNER || j < m) ? s : 0.0f;
-        }
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int k = k0 + c;  // newest loaded slot
-            if (k >= 1) {
-                // smooth slot k - 1
-                const float l0 = from_left<G>(v_prev, v_cur), r0 = from_right<G>(v_cur, vin[c]);
-                float sm_new = 0.25f * l0 + 0.5f * v_cur + 0.25f * r0;
-                if (!INNER) {
-                    const int j = gl + G * (k - 1);
-                    sm_new = (j >= 1 && j + 1 < m) ? sm_new : v_cur;
+                for (int q = 0; q < 4; ++q) {
+                    const int j = p + q;
+                    const bool in = FULL || j < m;
+                    float yj = 0.0f, xj = grid.left_x;
+                    if (in) {
+                        yj = y[j];
+                        if (NZ)
+                            xj = x[j];
+                    }
+                    const float v = NZ ? yj - (level + 0.0f * (xj - grid.left_x)) : yj - level;
+                    s[q] = in ? v : 0.0f;
                 }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                nan_acc = __builtin_fmaf(s[q], 0.0f, nan_acc);
+        };
+        // Smooths step kk (nq0: quarter of the next step's first point of this
+        // lane) and keeps the lane's largest value over 1 ≤ j ≤ M − 2.
+        auto track = [&](int kk, float nq0, auto inner_tag) {
+            constexpr bool INNER = decltype(inner_tag)::value;
+            const float lq = from_left<G>(pq3, cq[3]);  // quarter of point p − 1
+            const float rq = from_right<G>(cq[0], nq0);  // quarter of point p + 4
+            float sm[4];
+            sm[0] = (lq + ch[0]) + cq[1];
+            sm[1] = (cq[0] + ch[1]) + cq[2];
+            sm[2] = (cq[1] + ch[2]) + cq[3];
+            sm[3] = (cq[2] + ch[3]) + rq;
+            if (!INNER) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = 64 * kk + 4 * gl + q;
+                    sm[q] = (j >= 1 && j + 1 < m) ? sm[q] : -__builtin_inff();
+                }
+            }
+            const float mx = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+            if (mx > lane_best) {
+                lane_best = mx;
+                lane_step = kk;
+            }
+        };
+        auto shift_in = [&](const float(&s)[4]) {
+            pq3 = cq[3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cq[q] = 0.25f * s[q];
+                ch[q] = 0.5f * s[q];
+            }
+        };
+        // Steps k0 … k0 + CS − 1 are loaded and k0 − 1 … k0 + CS − 2 smoothed.
+        // INNER: every step loaded lies in the row and every point smoothed
+        // in 1 … M − 2 (k0 ≥ 2 and 64·(k0 + CS) ≤ M).
+        auto chunk4 = [&](int k0, auto inner_tag, auto vec_tag, auto nz_tag) {
+            float s[CS][4];
+#pragma unroll
+            for (int c = 0; c < CS; ++c)
+                load4(k0 + c, s[c], inner_tag, vec_tag, nz_tag);
+#pragma unroll
+            for (int c = 0; c < CS; ++c) {
+                if (k0 + c >= 1)
+                    track(k0 + c - 1, 0.25f * s[c][0], inner_tag);
+                shift_in(s[c]);
+            }
+        };
+        using yes = std::true_type;
+        using no = std::false_type;
+        const bool vec = !neg_zero && (reinterpret_cast<uintptr_t>(y) & 15u) == 0 && (m & 3) == 0;
+        int k0 = 0;
+        for (; k0 < steps; k0 += CS) {
+            const bool inner = k0 >= 2 && 64 * (k0 + CS) <= m;
+            if (vec)
+                inner ? chunk4(k0, yes{}, yes{}, no{}) : chunk4(k0, no{}, yes{}, no{});
+            else if (neg_zero)
+                chunk4(k0, no{}, no{}, yes{});
+            else
+                inner ? chunk4(k0, yes{}, no{}, no{}) : chunk4(k0, no{}, no{}, no{});
+        }
+        track(k0 - 1, 0.0f, no{});  // the last step loaded: nothing beyond it
+        sbad = group_or<G>(!finite_f(nan_acc));
+        const float top = group_max_f<G>(lane_best);
+        uint32_t cand = 0xffffffffu;
+        if (lane_step >= 0 && lane_best == top) {
+            // This lane's four points of that step, from its six signal values
+            // (the same operations as the pass), the first equal to the top.
+            const int p = 64 * lane_step + 4 * gl;
+            float sv[6];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const int j = p - 1 + d;
+                sv[d] = (j >= 0 && j < m) ? sig(y[j], neg_zero ? x[j] : 0.0f) : 0.0f;
+            }
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                const float smq = (0.25f * sv[q] + 0.5f * sv[q + 1]) + 0.25f * sv[q + 2];
+                const int j = p + q;
+                if (j >= 1 && j + 1 < m && smq == top)
+                    cand = uint32_t(j);
+            }
+        }
+        cand = group_min_u<G>(cand);
+        if (top > 0.0f && cand != 0xffffffffu) {
+            best = top;
+            first = int(cand);
+        }
+    }
+    // A first maximum at either end of 1 … M − 2 is the reference's only when
+    // it beats the row's end point (smooth() returns end points as they are).
+    bool replay = false;
+    if (!sbad && first == 1)
+        replay = !(best > sig(y[0], neg_zero ? x[0] : 0.0f));
+    else if (!sbad && first == m - 2)
+        replay = !(best >= sig(y[m - 1], neg_zero ? x[m - 1] : 0.0f));
+    if (replay) {
+        // The reference's scan (initial_guess.hpp:170-182) point by point:
+        // 16 points per slot, one per lane; the slot being tested lags the
+        // slot being smoothed by one, which lags the newest loaded by one.
+        best = 0.0f;
+        first = 0x7fffffff;
+        const int slots = (m + G - 1) / G;
+        float v_prev = 0.0f, v_cur = 0.0f, sm_prev = 0.0f, sm_cur = 0.0f;
+        auto slot_step = [&](int k, float vin) {  // vin: slot k's value
+            if (k >= 1) {  // smooth slot k - 1
+                const float l0 = from_left<G>(v_prev, v_cur), r0 = from_right<G>(v_cur, vin);
+                float sm_new = 0.25f * l0 + 0.5f * v_cur + 0.25f * r0;
+                const int jn = gl + G * (k - 1);
+                sm_new = (jn >= 1 && jn + 1 < m) ? sm_new : v_cur;
                 if (k >= 2) {  // test slot k - 2
                     const float l = from_left<G>(sm_prev, sm_cur), r = from_right<G>(sm_cur, sm_new);
-                    bool in = true;
-                    if (!INNER) {
-                        const int j = gl + G * (k - 2);
-                        in = j >= 1 && j + 1 < m;
-                    }
-                    if (in && sm_cur > (l > best ? l : best) && sm_cur >= r) {
+                    const int j = gl + G * (k - 2);
+                    if (j >= 1 && j + 1 < m && sm_cur > (l > best ? l : best) && sm_cur >= r) {
                         best = sm_cur;
-                        first = gl + G * (k - 2);
+                        first = j;
                     }
                 }
                 sm_prev = sm_cur;
                 sm_cur = sm_new;
             }
             v_prev = v_cur;
-            v_cur = vin[c];
-        }
-    };
-    using yes = std::true_type;
-    using no = std::false_type;
-    // A chunk is inner when the slots it smooths and tests (k0 - 2 … k0 + CH
-    // - 2) hold only points 1 … M - 2 and the slots it loads lie in the row.
-    for (int k0 = 0; k0 < slots; k0 += CH) {
-        const bool inner = k0 >= 3 && G * (k0 + CH) <= m - 1;
-        if (neg_zero)
-            inner ? chunk(k0, yes{}, yes{}) : chunk(k0, no{}, yes{});
-        else
-            inner ? chunk(k0, yes{}, no{}) : chunk(k0, no{}, no{});
-    }
-    {
-        // Drain: the last slot loaded is smoothed with zeros beyond, and the
-        // last two are tested. Slots past the row are zeros and never tested.
-        const int k = slots + ((slots % CH) ? CH - slots % CH : 0);  // slots loaded
-        const float l0 = from_left<G>(v_prev, v_cur), r0 = from_right<G>(v_cur, 0.0f);
-        float sm_new = 0.25f * l0 + 0.5f * v_cur + 0.25f * r0;
-        {
-            const int j = gl + G * (k - 1);
-            sm_new = (j >= 1 && j + 1 < m) ? sm_new : v_cur;
-        }
-        auto test = [&](int kt, float smp, float smc, float smn) {
-            const float l = from_left<G>(smp, smc), r = from_right<G>(smc, smn);
-            const int j = gl + G * kt;
-            if (j >= 1 && j + 1 < m && smc > (l > best ? l : best) && smc >= r) {
-                best = smc;
-                first = j;
-            }
+            v_cur = vin;
         };
-        test(k - 2, sm_prev, sm_cur, sm_new);
-        test(k - 1, sm_cur, sm_new, 0.0f);
+        for (int k = 0; k < slots + 2; ++k) {  // two slots of zeros drain the lag
+            const int j = gl + G * k;
+            slot_step(k, (k < slots && j < m) ? sig(y[j], neg_zero ? x[j] : 0.0f) : 0.0f);
+        }
+        group_argmax<G>(best, first);
     }
-    group_argmax<G>(best, first);
-    ybad = group_or<G>(ybad);
-    sbad = group_or<G>(sbad);
+    int ybad = 0;
+    if (sbad) {  // which kind of non-finite value: an observation's or the signal's
+        for (int j = gl; j < m; j += G)
+            ybad |= !finite_f(y[j]);
+        ybad = group_or<G>(ybad);
+    }
     if (ybad) {  // non-finite observation: no baseline either (initial_guess.hpp:131-139)
         write();
         return;
@@ -704,8 +996,18 @@ bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
         const uint32_t count = a.m / 10 > 1 ? a.m / 10 : 1;
         if (!guess_group_serves(MODEL, a.ys_pt) || a.n == 0)
             return false;
-        if (count <= 112)
-            e = launch_guess_levels<16, 7>(a, P, stream);
+        const uint32_t sorted_blocks = (a.n + 31) / 32;
+        if (count <= 32)
+            hipLaunchKernelGGL(guess_levels_sort_kernel<4>, dim3(sorted_blocks), dim3(256), 0,
+                               stream, a, P);
+        else if (count <= 64)
+            hipLaunchKernelGGL(guess_levels_sort_kernel<8>, dim3(sorted_blocks), dim3(256), 0,
+                               stream, a, P);
+        else if (count <= 128)
+            hipLaunchKernelGGL(guess_levels_sort_kernel<16>, dim3(sorted_blocks), dim3(256), 0,
+                               stream, a, P);
+        if (count <= 128)
+            e = hipGetLastError();
         else if (count <= 448)
             e = launch_guess_levels<64, 7>(a, P, stream);
         else if (count <= 1024)

@@ -558,3 +558,46 @@ def test_two_peak_guess_length_limit(hip):
     x = np.linspace(-5, 5, 12289).astype(np.float32)
     with pytest.raises(ValueError, match="12,288"):
         hip.fit(x, np.ones((2, x.size), np.float32), "two_lorentzian", None, None, "hip")
+
+
+def peak_scan_rows(m, rng):
+    """Rows aimed at the single-peak scan's corners: the largest smoothed value
+    at j = 1 or M − 2 with the end point above it (the reference then takes
+    another local maximum) or below it, flat tops (ties: the first index
+    wins), a level of −0 (the signal reads the grid), non-finite observations
+    and a signal that overflows, no positive maximum, and plain noisy peaks."""
+    x = np.linspace(-5, 5, m)
+    bump = np.exp(-0.5 * (x / 0.4) ** 2)
+    rows = []
+    ramp = np.linspace(10, 0, m)
+    rows += [ramp + 0.5 * bump, ramp[::-1] + 0.5 * bump, ramp + 3 * bump, ramp[::-1] + 3 * bump]
+    r = np.linspace(4, 0, m)
+    r[:3] = [5, 10, 4]
+    rows += [r, r[::-1].copy()]
+    flat = np.zeros(m)
+    flat[m // 3: m // 3 + 5] = 2.0
+    rows += [flat, np.round(3 * bump, 1)]
+    neg = np.full(m, -0.0)
+    neg[m // 2 - 2: m // 2 + 3] = [0.5, 1.5, 2.0, 1.5, 0.5][: min(5, m)]
+    rows += [neg, -bump]
+    big = np.full(m, -3e38)
+    big[m // 2] = 3e38
+    nan = 1 + bump
+    nan[m // 2] = np.nan
+    inf = 1 + bump
+    inf[0] = np.inf
+    rows += [big, nan, inf]
+    rows += [1 + bump + 0.05 * rng.standard_normal(m) for _ in range(5)]
+    return x.astype(np.float32), np.array(rows, np.float32)
+
+
+@pytest.mark.parametrize("m", [5, 6, 7, 8, 63, 64, 65, 67, 68, 255, 256, 257, 1023, 1024, 1028])
+def test_guess_peak_scan_corners(hip, oracle_port, m):
+    rng = np.random.default_rng(m)
+    x, y = peak_scan_rows(m, rng)
+    for model in ("gaussian", "lorentzian", "pseudo_voigt"):
+        got = device_guess(x, y, model)
+        ref = oracle_port.guess(x, y, model)
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                                  np.ascontiguousarray(b).view(np.uint8)), (model, m, a, b)
