@@ -1,14 +1,15 @@
 // bf_guess_group.hpp — data-only initial guess of the single-peak models
 // (Gaussian, Lorentzian, pseudo-Voigt; cpu/initial_guess.hpp:120-197) in two
-// stages with several fits per wave: the edge levels by groups of 16 or 64
-// lanes, then the peak by groups of 16 lanes, streaming the row. Equal to the
-// FP32 reference guess bit for bit, like the one-wave-per-fit kernel of
-// bf_guess.hpp, which still serves the two-peak model, edge windows over 1024
-// points and point-major data. No LDS; few registers per lane.
+// stages with several fits per wave: the edge levels (sorting networks in
+// 8-lane groups for edge windows up to 128 values, bisection in 64-lane groups
+// up to 1024, one wave per fit reading the keys from memory beyond), then the
+// peak by groups of 8 lanes, streaming the row. Equal to the FP32 reference
+// guess bit for bit, like the one-wave-per-fit kernel of bf_guess.hpp, which
+// still serves the two-peak model and point-major data. No LDS.
 //
-// The edge medians are exact order statistics found by bisection with each
-// group's own search state in VGPRs, so the groups of a wave search at once
-// and the wave runs as many steps as its slowest median.
+// The bisection medians are exact order statistics with each group's own
+// search state in VGPRs, so the groups of a wave search at once and the wave
+// runs as many steps as its slowest median.
 #pragma once
 
 #include "bf_guess.hpp"
@@ -50,7 +51,7 @@ template <int G> __device__ __forceinline__ int group_sum_i(int v) {
 }
 // Largest value of the group, ties to the smallest index (finite values).
 template <int G> __device__ __forceinline__ void group_argmax(float &v, int &idx) {
-    static_assert(G >= 16 && G <= 64, "groups of 16 to 64 lanes");
+    static_assert(G >= 8 && G <= 64, "groups of 8 to 64 lanes");
     auto step = [&](float w, int wi) {
         const bool take = w > v || (w == v && wi < idx);
         v = take ? w : v;
@@ -59,7 +60,8 @@ template <int G> __device__ __forceinline__ void group_argmax(float &v, int &idx
     step(__int_as_float(xlane<1>(__float_as_int(v))), xlane<1>(idx));
     step(__int_as_float(xlane<2>(__float_as_int(v))), xlane<2>(idx));
     step(__int_as_float(xlane<4>(__float_as_int(v))), xlane<4>(idx));
-    step(__int_as_float(xlane<8>(__float_as_int(v))), xlane<8>(idx));
+    if constexpr (G >= 16)
+        step(__int_as_float(xlane<8>(__float_as_int(v))), xlane<8>(idx));
     if constexpr (G >= 32)
         step(__int_as_float(xlane<16>(__float_as_int(v))), xlane<16>(idx));
     if constexpr (G >= 64)
@@ -243,8 +245,8 @@ __device__ void mean_bracket(const float (&val)[S], const bool (&valid)[S], int 
 
 // DPP moves of one float: ctrl codes of gfx9 (row_shl:1 0x101, row_shr:1
 // 0x111, row_ror:1 0x121, row_ror:15 0x12F, wave_shl:1 0x130, wave_rol:1
-// 0x134, wave_shr:1 0x138, wave_ror:1 0x13C). A lane whose source lies
-// outside its row (or the wave) keeps `old`.
+// 0x134, wave_shr:1 0x138, wave_ror:1 0x13C, row_half_mirror 0x141). A lane
+// whose source lies outside its row (or the wave) keeps `old`.
 template <int CTRL> __device__ __forceinline__ float dpp_f(float old, float src) {
     return __int_as_float(
         __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xF, 0xF, false));
@@ -255,19 +257,29 @@ template <int CTRL> __device__ __forceinline__ float dpp_rot(float src) {
 }
 // The value at point j − 1 for the point of slot k of every lane (cur = slot
 // k, prev = slot k − 1): from the lane below, or for the group's first lane
-// from the group's last lane one slot earlier.
+// from the group's last lane one slot earlier. Groups of 8 lanes share a DPP
+// row, so their first lane takes row_half_mirror of prev (lane 7 of its
+// group) by a select.
 template <int G> __device__ __forceinline__ float from_left(float prev, float cur) {
-    if constexpr (G == 16)
+    if constexpr (G == 8) {
+        const float wrap = dpp_rot<0x141>(prev), below = dpp_rot<0x111>(cur);
+        return (threadIdx.x & 7) == 0 ? wrap : below;
+    } else if constexpr (G == 16) {
         return dpp_f<0x111>(dpp_rot<0x121>(prev), cur);
-    else
+    } else {
         return dpp_f<0x138>(dpp_rot<0x13C>(prev), cur);
+    }
 }
 // The value at point j + 1 (cur = slot k, next = slot k + 1).
 template <int G> __device__ __forceinline__ float from_right(float cur, float next) {
-    if constexpr (G == 16)
+    if constexpr (G == 8) {
+        const float wrap = dpp_rot<0x141>(next), above = dpp_rot<0x101>(cur);
+        return (threadIdx.x & 7) == 7 ? wrap : above;
+    } else if constexpr (G == 16) {
         return dpp_f<0x101>(dpp_rot<0x12F>(next), cur);
-    else
+    } else {
         return dpp_f<0x130>(dpp_rot<0x134>(next), cur);
+    }
 }
 
 // Stage 1: the edge levels of a fit (initial_guess.hpp:142-169) by a group of
@@ -495,13 +507,11 @@ template <int S> __device__ __forceinline__ uint32_t key_at8(const uint32_t (&v)
     return group_max_u<8>(gl == e / S ? t[0] : 0u);
 }
 
+// Level and noise of fit f by its 8-lane group (every lane gets them).
 template <int S>
-__global__ void __launch_bounds__(256) guess_levels_sort_kernel(GuessArgs a, int param_stride) {
+__device__ __forceinline__ void sort_levels(const GuessArgs &a, uint32_t f, int gl, float &level_out,
+                                            float &noise_out) {
     constexpr int G = 8, N = G * S;
-    const int gl = int(threadIdx.x & (G - 1));
-    const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
-    const bool live = fit < a.n;
-    const uint32_t f = live ? fit : a.n - 1;
     const int m = int(a.m);
     const int c = a.grid->count;  // 1 … N
     const float *y = a.y + size_t(f) * size_t(a.ys_fit);
@@ -582,38 +592,33 @@ __global__ void __launch_bounds__(256) guess_levels_sort_kernel(GuessArgs a, int
     const uint32_t d_lo = key_at8<S>(ld, c - 1, gl);
     const uint32_t d_hi = c == N ? top : key_at8<S>(ld, c, gl);
     const float spread = std_midpoint(key_float(d_lo), key_float(d_hi));
-    if (live && gl == 0) {
-        a.params[size_t(fit) * param_stride] = level;
-        a.params[size_t(fit) * param_stride + 1] = 1.4826f * spread;
-    }
+    level_out = level;
+    noise_out = 1.4826f * spread;
 }
 
-// Stage 2 of the guess, one fit per group of 16 lanes (four per wave): lane gl
-// takes the points gl, gl + 16, … as it streams the row in chunks of 16·CH
-// points (coalesced; a sliding window of slots, so the registers do not grow
-// with M). The smoothed-maximum scan takes each point's neighbours from the
-// adjacent lanes with DPP (row shifts; the group's edge lanes from the
-// neighbouring slot by a row rotate). The half-height searches of peak() walk
-// outwards from the peak 16 points per step (a few steps: a half width is
-// tens of points), and the centroid and the crossings read the few points
-// they need in one batch. Level and noise come from stage 1 through the
-// fit's parameter slots, which this stage then overwrites.
-template <int MODEL>
-__global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
+// Stage 2 of the guess, one fit per group of G = 8 or 16 lanes (eight or four
+// per wave), streaming the row (a sliding window, so the registers do not grow
+// with M). The smoothed-maximum scan is described at the scan. The
+// half-height searches of peak() walk outwards from the peak G points per step
+// (a few steps: a half width is tens of points), and the centroid and the
+// crossings read the few points they need in one batch. Level and noise come
+// from stage 1 through the fit's parameter slots, which this stage then
+// overwrites.
+template <int MODEL, int G>
+__device__ __forceinline__ void peak_stage(const GuessArgs &a, uint32_t fit, float level,
+                                           float noise) {
     static_assert(MODEL != TWO_LORENTZIAN, "the two-peak guess runs one wave per fit");
-    constexpr int G = 16;
+    static_assert(G == 8 || G == 16, "groups of 8 or 16 lanes");
+    constexpr int W = 4 * G;  // points per step of the scan
     constexpr int P = Params<MODEL>::P;
     const int lane = int(threadIdx.x & 63);
     const int gl = lane & (G - 1);
-    const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
     const bool live = fit < a.n;
     const uint32_t f = live ? fit : a.n - 1;
     const int m = int(a.m);
     const float *y = a.y + size_t(f) * size_t(a.ys_fit);
     const float *x = a.x;
     const GridInfo grid = *a.grid;
-    const float level = a.params[size_t(f) * P];
-    const float noise = a.params[size_t(f) * P + 1];
     float out[P];
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -647,7 +652,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     // before it is as large, nothing after it larger), unless it sits at j = 1
     // or M − 2 and the row's end point beats it: then, and only then, the
     // reference's scan is replayed point by point (replay_scan). The fast pass
-    // streams the row 64 points per step, four consecutive points per lane
+    // streams the row 4·G points per step, four consecutive points per lane
     // (one 16-byte load when the rows are 16-byte aligned), so a point's
     // neighbours sit in its own lane except at the lane's two ends (DPP).
     // Each lane keeps its largest smoothed value and the first step that
@@ -658,7 +663,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     int sbad = 0;
     {
         constexpr int CS = 4;  // steps per chunk (loads in flight)
-        const int steps = (m + 63) / 64;
+        const int steps = (m + W - 1) / W;
         float lane_best = -__builtin_inff();
         int lane_step = -1;
         float nan_acc = 0.0f;  // Σ s·0: NaN once a signal value is not finite
@@ -669,7 +674,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
         auto load4 = [&](int k, float(&s)[4], auto full_tag, auto vec_tag, auto nz_tag) {
             constexpr bool FULL = decltype(full_tag)::value, VEC = decltype(vec_tag)::value;
             constexpr bool NZ = decltype(nz_tag)::value;
-            const int p = 64 * k + 4 * gl;
+            const int p = W * k + 4 * gl;
             if constexpr (VEC) {  // M is a multiple of 4: a load is all in or all out
                 float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 const bool in = FULL || p < m;
@@ -712,7 +717,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
             if (!INNER) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int j = 64 * kk + 4 * gl + q;
+                    const int j = W * kk + 4 * gl + q;
                     sm[q] = (j >= 1 && j + 1 < m) ? sm[q] : -__builtin_inff();
                 }
             }
@@ -732,7 +737,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
         };
         // Steps k0 … k0 + CS − 1 are loaded and k0 − 1 … k0 + CS − 2 smoothed.
         // INNER: every step loaded lies in the row and every point smoothed
-        // in 1 … M − 2 (k0 ≥ 2 and 64·(k0 + CS) ≤ M).
+        // in 1 … M − 2 (k0 ≥ 2 and W·(k0 + CS) ≤ M).
         auto chunk4 = [&](int k0, auto inner_tag, auto vec_tag, auto nz_tag) {
             float s[CS][4];
 #pragma unroll
@@ -750,7 +755,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
         const bool vec = !neg_zero && (reinterpret_cast<uintptr_t>(y) & 15u) == 0 && (m & 3) == 0;
         int k0 = 0;
         for (; k0 < steps; k0 += CS) {
-            const bool inner = k0 >= 2 && 64 * (k0 + CS) <= m;
+            const bool inner = k0 >= 2 && W * (k0 + CS) <= m;
             if (vec)
                 inner ? chunk4(k0, yes{}, yes{}, no{}) : chunk4(k0, no{}, yes{}, no{});
             else if (neg_zero)
@@ -765,7 +770,7 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
         if (lane_step >= 0 && lane_best == top) {
             // This lane's four points of that step, from its six signal values
             // (the same operations as the pass), the first equal to the top.
-            const int p = 64 * lane_step + 4 * gl;
+            const int p = W * lane_step + 4 * gl;
             float sv[6];
 #pragma unroll
             for (int d = 0; d < 6; ++d) {
@@ -880,11 +885,12 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
         return;
     }
     const float half = amplitude / 2.0f;
-    // The nearest point at or below half on each side: 16 points per step
+    // The nearest point at or below half on each side: G points per step
     // outwards from the peak, ending at the first step that holds one (or at
     // the row's end: then the end point, which is above half).
     int lft = -1, rgt = -1;
     const int shift = lane & ~(G - 1);
+    constexpr uint32_t kGroupMask = (1u << G) - 1u;
     for (int t = 0;; ++t) {
         const bool want_l = lft < 0, want_r = rgt < 0;
         if (!__ballot(want_l || want_r))
@@ -895,8 +901,8 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
             cl = sig(y[jl], neg_zero ? x[jl] : 0.0f) <= half;
         if (want_r && jr < m)
             cr = sig(y[jr], neg_zero ? x[jr] : 0.0f) <= half;
-        const uint32_t ml = uint32_t(__ballot(cl) >> shift) & 0xffffu;
-        const uint32_t mr = uint32_t(__ballot(cr) >> shift) & 0xffffu;
+        const uint32_t ml = uint32_t(__ballot(cl) >> shift) & kGroupMask;
+        const uint32_t mr = uint32_t(__ballot(cr) >> shift) & kGroupMask;
         if (want_l) {
             if (ml)
                 lft = first - 1 - t * G - __builtin_ctz(ml);
@@ -971,6 +977,23 @@ __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
     write();
 }
 
+template <int MODEL, int G> __global__ void __launch_bounds__(256) guess_peak_kernel(GuessArgs a) {
+    constexpr int P = Params<MODEL>::P;
+    const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const uint32_t f = fit < a.n ? fit : a.n - 1;
+    peak_stage<MODEL, G>(a, fit, a.params[size_t(f) * P], a.params[size_t(f) * P + 1]);
+}
+
+// Both stages in one launch for edge windows up to 8·S values: a wave sorts
+// the edges of its eight fits (VALU work) and streams their rows (memory
+// work), so the waves of a SIMD overlap one with the other.
+template <int MODEL, int S> __global__ void __launch_bounds__(256) guess_single_kernel(GuessArgs a) {
+    const uint32_t fit = blockIdx.x * 32 + threadIdx.x / 8;
+    float level, noise;
+    sort_levels<S>(a, fit < a.n ? fit : a.n - 1, int(threadIdx.x & 7), level, noise);
+    peak_stage<MODEL, 8>(a, fit, level, noise);
+}
+
 template <int G, int CE> hipError_t launch_guess_levels(const GuessArgs &a, int stride,
                                                         hipStream_t stream) {
     const uint32_t per_block = 256 / G;
@@ -996,19 +1019,21 @@ bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
         const uint32_t count = a.m / 10 > 1 ? a.m / 10 : 1;
         if (!guess_group_serves(MODEL, a.ys_pt) || a.n == 0)
             return false;
-        const uint32_t sorted_blocks = (a.n + 31) / 32;
-        if (count <= 32)
-            hipLaunchKernelGGL(guess_levels_sort_kernel<4>, dim3(sorted_blocks), dim3(256), 0,
-                               stream, a, P);
-        else if (count <= 64)
-            hipLaunchKernelGGL(guess_levels_sort_kernel<8>, dim3(sorted_blocks), dim3(256), 0,
-                               stream, a, P);
-        else if (count <= 128)
-            hipLaunchKernelGGL(guess_levels_sort_kernel<16>, dim3(sorted_blocks), dim3(256), 0,
-                               stream, a, P);
-        if (count <= 128)
+        const uint32_t blocks8 = (a.n + 31) / 32;  // 8 lanes per fit
+        if (count <= 128) {  // both stages in one launch
+            if (count <= 32)
+                hipLaunchKernelGGL((guess_single_kernel<MODEL, 4>), dim3(blocks8), dim3(256), 0,
+                                   stream, a);
+            else if (count <= 64)
+                hipLaunchKernelGGL((guess_single_kernel<MODEL, 8>), dim3(blocks8), dim3(256), 0,
+                                   stream, a);
+            else
+                hipLaunchKernelGGL((guess_single_kernel<MODEL, 16>), dim3(blocks8), dim3(256), 0,
+                                   stream, a);
             e = hipGetLastError();
-        else if (count <= 448)
+            return true;
+        }
+        if (count <= 448)
             e = launch_guess_levels<64, 7>(a, P, stream);
         else if (count <= 1024)
             e = launch_guess_levels<64, 16>(a, P, stream);
@@ -1019,8 +1044,7 @@ bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
         }
         if (e != hipSuccess)
             return true;
-        hipLaunchKernelGGL((guess_peak_kernel<MODEL>), dim3((a.n + 15) / 16), dim3(256), 0,
-                           stream, a);
+        hipLaunchKernelGGL((guess_peak_kernel<MODEL, 8>), dim3(blocks8), dim3(256), 0, stream, a);
         e = hipGetLastError();
         return true;
     }

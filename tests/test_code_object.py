@@ -60,14 +60,14 @@ def resources(tmp_path_factory):
 def test_guess_kernels_use_no_private_memory(resources):
     guess = {k: v for k, v in resources.items() if "guess_kernel" in k or "guess_ragged_kernel" in k}
     assert len(guess) == 8, sorted(guess)  # uniform and ragged, four models
-    # The two-stage guess of the single-peak models: edge levels (sorting
-    # networks for windows up to 32, 64 and 128 values, bisection in 64-lane
-    # groups up to 1024, one wave per fit with the keys read from memory
-    # beyond) and the peak (three models).
+    # The guess of the single-peak models: one kernel per model for edge
+    # windows up to 32, 64 and 128 values (sorting networks, then the peak),
+    # and beyond, edge levels (bisection in 64-lane groups up to 1024, one
+    # wave per fit with the keys read from memory beyond) then the peak.
     staged = {k: v for k, v in resources.items()
               if "guess_levels_kernel" in k or "guess_levels_stream_kernel" in k
-              or "guess_levels_sort_kernel" in k or "guess_peak_kernel" in k}
-    assert len(staged) == 9, sorted(staged)
+              or "guess_single_kernel" in k or "guess_peak_kernel" in k}
+    assert len(staged) == 15, sorted(staged)
     guess.update(staged)
     for name, r in guess.items():
         assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)
