@@ -26,7 +26,10 @@ of amplitude, seed 7, p0=None, bounds [0,-10,1e-4,0]–[inf,10,10,1]
                degenerate rows (seed 11);
 * latency    — small batches (32-2048 fits, M 1024-8192): the throughput
                table against the latency mapping (workgroups per fit);
-* cpu_baseline — the reference's own CPU path on a bounded sample.
+* cpu_baseline — the reference's own CPU path on a bounded sample (float32
+               and float64, all cores and one worker), the GPU's metric rows
+               against its float64 results, and 4,096 overlapping-doublet rows
+               (config 3) against float64.
 
 ``--only`` picks sections (production is always run unless
 ``--only calibrate``); ``--calibrate-copy`` times a plain device copy of
@@ -732,6 +735,54 @@ def fp64_check(ref, gpu_params, gpu_status, gpu_chi2):
                     "chi2 <= 1e-5 relative"}
 
 
+STATUS_NAMES = ["converged", "max_iterations", "singular_matrix", "invalid_input",
+                "degenerate_data", "unresolved_peak"]
+
+
+def doublets_vs_fp64(lib, threads, rows=4096):
+    """Overlapping doublets (tests/golden/datasets.py:config3, first ``rows``
+    rows, p0=None): the GPU's statuses against the reference's float64 fit of
+    the same float32 values, beside the reference's own float32 fit against
+    its float64 fit (the agreement FP32 arithmetic can reach on these rows),
+    and the GPU against float64 run at the float32 tolerances."""
+    import collections
+    import batchfit_amd
+    from oracle import oracle
+    import datasets
+    x, y, bounds = datasets.config3(rows=rows)[:3]
+    got = batchfit_amd.fit(x, y, "two_lorentzian", None, bounds, device="hip")
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
+    r32 = lib.fit(x, y, "two_lorentzian", None, bounds, threads=threads)
+    r64 = lib.fit(x64, y64, "two_lorentzian", None, bounds, threads=threads)
+    r64t = lib.fit(x64, y64, "two_lorentzian", None, bounds, threads=threads,
+                   options=oracle.make_options(gradient_tolerance=1e-5, step_tolerance=1e-6,
+                                               cost_tolerance=1e-6))
+    off = got.status != r64.status
+    classes = collections.Counter(f"{STATUS_NAMES[a]} vs {STATUS_NAMES[b]}"
+                                  for a, b in zip(got.status[off], r64.status[off]))
+    both = (got.status == 0) & (r64.status == 0)
+    g, r = got.params[both].astype(np.float64), r64.params[both]
+    err = {"amplitudes": np.abs(g[:, [0, 3]] - r[:, [0, 3]]) / np.abs(r[:, [0, 3]]),
+           "widths": np.abs(g[:, [2, 5]] - r[:, [2, 5]]) / np.abs(r[:, [2, 5]]),
+           "centres_over_width": np.abs(g[:, [1, 4]] - r[:, [1, 4]]) / np.abs(r[:, [2, 5]]),
+           "chi2": np.abs(got.chi2[both] - r64.chi2[both]) / r64.chi2[both]}
+    out = {"rows": rows,
+           "gpu_vs_fp64": float(np.mean(~off)),
+           "reference_float32_vs_fp64": float(np.mean(r32.status == r64.status)),
+           "gpu_vs_reference_float32": float(np.mean(got.status == r32.status)),
+           "gpu_vs_fp64_at_float32_tolerances": float(np.mean(got.status == r64t.status)),
+           "disagreement_classes_gpu_vs_fp64": dict(classes.most_common()),
+           "converged_in_both": int(both.sum()),
+           "median_error_vs_fp64": {k: float(np.median(v)) if v.size else None
+                                    for k, v in err.items()},
+           "max_error_vs_fp64": {k: float(v.max()) if v.size else None for k, v in err.items()}}
+    log(f"doublets vs FP64 ({rows} rows): GPU {out['gpu_vs_fp64']:.4f}, reference float32 "
+        f"{out['reference_float32_vs_fp64']:.4f}, GPU vs FP64 at the float32 tolerances "
+        f"{out['gpu_vs_fp64_at_float32_tolerances']:.4f}, GPU vs reference float32 "
+        f"{out['gpu_vs_reference_float32']:.4f}")
+    return out
+
+
 def cpu_baseline(x, y, bounds, gpu=None):
     """The reference's own CPU path (oracle/_ref: cpu::initial_guess + fit_batch
     over its ThreadPool) on the first rows of the metric workload, p0=None,
@@ -772,6 +823,7 @@ def cpu_baseline(x, y, bounds, gpu=None):
                      "after a 256-fit warm-up; float64 legs fit the same float32 values "
                      "promoted",
            "legs_fits_per_s": legs, "host": host_cpu()}
+    out["doublets_vs_fp64"] = doublets_vs_fp64(lib, cores)
     if gpu is not None:
         params, status, chi2 = gpu
         n = y.shape[0]
