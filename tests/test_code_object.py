@@ -1,7 +1,7 @@
 """Kernel resources of the built library, read from its gfx950 code objects
 (no device needed): the device guess kernels keep everything in registers
-and LDS (no private segment: no stack frames, no out-of-line calls whose
-arguments live in scratch), and the production LM kernel of the metric
+and LDS (no private segment and no dynamic stack: no stack frames, no
+out-of-line calls whose arguments live in scratch), and the production LM kernel of the metric
 workload (pseudo-Voigt, 8 lanes, streamed, compensated, FP64 solve, guessed
 starts) neither spills nor uses scratch."""
 import os
@@ -44,6 +44,8 @@ def kernel_resources(tmp_path):
                 kernels[name]["private"] = int(line.split(":")[1])
             elif name and line.startswith(".vgpr_spill_count:"):
                 kernels[name]["spill"] = int(line.split(":")[1])
+            elif name and line.startswith(".uses_dynamic_stack:"):
+                kernels[name]["dynamic_stack"] = line.split(":")[1].strip() == "true"
     return kernels
 
 
@@ -71,6 +73,7 @@ def test_guess_kernels_use_no_private_memory(resources):
     guess.update(staged)
     for name, r in guess.items():
         assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)
+        assert not r.get("dynamic_stack", False), (name, r)
 
 
 def test_metric_lm_kernel_has_no_spills(resources):
