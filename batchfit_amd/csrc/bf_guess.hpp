@@ -42,6 +42,9 @@ __host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
 // Largest M the device guess takes: the signal and the sort buffer of its
 // edge medians fit in 64 KiB of LDS (12288 + 4096 floats).
 constexpr uint32_t kGuessMaxPoints = 12288;
+// Largest M whose edge windows (M / 10 values each) the sorting-network guess
+// of the single-peak models holds: 128 values, eight lanes of 16 slots.
+constexpr uint32_t kGuessSortedMaxPoints = 1289;
 
 // LDS floats the guess of `model` needs beyond the signal of M points: the
 // sort buffer of its edge medians and, for the two-peak model, one minimum

@@ -346,7 +346,8 @@ __global__ void __launch_bounds__(256) guess_levels_kernel(GuessArgs a, int para
 // Stage 1 for edge windows beyond 1024 values (M ≥ 10,250): one wave per fit,
 // the same searches with the window's keys read from memory at every step
 // (the two edges stay in the L2 cache), so M has no limit.
-__global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a, int param_stride) {
+template <int P> __global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a) {
+    constexpr int param_stride = P;
     const uint32_t fit = blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = int(threadIdx.x & 63);
     const bool live = fit < a.n;
@@ -1038,8 +1039,8 @@ bool launch_guess_group(const GuessArgs &a, hipStream_t stream, hipError_t &e) {
         else if (count <= 1024)
             e = launch_guess_levels<64, 16>(a, P, stream);
         else {
-            hipLaunchKernelGGL(guess_levels_stream_kernel, dim3((a.n + 3) / 4), dim3(256), 0,
-                               stream, a, P);
+            hipLaunchKernelGGL(guess_levels_stream_kernel<P>, dim3((a.n + 3) / 4), dim3(256), 0,
+                               stream, a);
             e = hipGetLastError();
         }
         if (e != hipSuccess)

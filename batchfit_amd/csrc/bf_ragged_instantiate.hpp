@@ -3,6 +3,7 @@
 // plain FP32 and compensated-mixed numerics).
 #pragma once
 
+#include "bf_guess_group.hpp"
 #include "bf_ragged.hpp"
 
 #include <algorithm>
@@ -12,6 +13,37 @@
 #endif
 
 namespace bfhip {
+
+// Data-only guess of single-peak fits of a ragged batch whose edge windows
+// hold at most 8·S values: one fit per group of 8 lanes, eight per wave, the
+// stages of guess_single_kernel on each fit's own row, grid and M.
+template <int MODEL, int S>
+__global__ void __launch_bounds__(256) guess_ragged_single_kernel(
+    const bf_fit_desc *fits, const uint32_t *order, uint32_t count, const bf_grid_desc *grids,
+    const GridInfo *grid_info, const float *x, const float *y, float *guess) {
+    const uint32_t task = blockIdx.x * 32 + threadIdx.x / 8;
+    if (task >= count)
+        return;  // the whole group: groups share no lanes' results
+    const bf_fit_desc d = fits[order[task]];
+    if (d.p0_row != 0xffffffffu)
+        return;
+    GuessArgs g;
+    g.x = x + grids[d.grid].x_offset;
+    g.y = y + d.y_offset;
+    g.ys_fit = 0;
+    g.ys_pt = 1;
+    g.grid = grid_info + d.grid;
+    g.n = 1;
+    g.m = d.points;
+    g.m_pad = 0;
+    g.sort_n = 0;
+    g.params = guess + size_t(d.out_row) * 8;
+    g.baseline = nullptr;
+    g.gstatus = nullptr;
+    float level, noise;
+    sort_levels<S>(g, 0, int(threadIdx.x & 7), level, noise);
+    peak_stage<MODEL, 8>(g, 0, level, noise);
+}
 
 namespace {
 template <int MODEL, int CLS, bool MIXED>
@@ -162,6 +194,17 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
     }
     if (e == hipSuccess && guess)
         e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&guess_ragged_kernel<BF_MODEL>));
+    if constexpr (BF_MODEL != TWO_LORENTZIAN) {
+        if (e == hipSuccess && guess)
+            e = hipFuncGetAttributes(
+                &attr, reinterpret_cast<const void *>(&guess_ragged_single_kernel<BF_MODEL, 4>));
+        if (e == hipSuccess && guess)
+            e = hipFuncGetAttributes(
+                &attr, reinterpret_cast<const void *>(&guess_ragged_single_kernel<BF_MODEL, 8>));
+        if (e == hipSuccess && guess)
+            e = hipFuncGetAttributes(
+                &attr, reinterpret_cast<const void *>(&guess_ragged_single_kernel<BF_MODEL, 16>));
+    }
     return e;
 }
 
@@ -177,6 +220,26 @@ hipError_t launch_ragged_guess<BF_MODEL>(const bf_fit_desc *fits, const uint32_t
                                          const GridInfo *grid_info, const float *x, const float *y,
                                          float *guess, uint32_t m_pad, uint32_t sort_n,
                                          hipStream_t stream) {
+    if constexpr (BF_MODEL != TWO_LORENTZIAN) {
+        // Single-peak fits of at most kGuessSortedMaxPoints: eight per wave
+        // (edge sorting networks, then the streamed peak), as a uniform batch.
+        if (m_pad <= kGuessSortedMaxPoints) {
+            const uint32_t edge = m_pad / 10 > 1 ? m_pad / 10 : 1, blocks = (count + 31) / 32;
+            if (edge <= 32)
+                hipLaunchKernelGGL((guess_ragged_single_kernel<BF_MODEL, 4>), dim3(blocks),
+                                   dim3(256), 0, stream, fits, order, count, grids, grid_info, x,
+                                   y, guess);
+            else if (edge <= 64)
+                hipLaunchKernelGGL((guess_ragged_single_kernel<BF_MODEL, 8>), dim3(blocks),
+                                   dim3(256), 0, stream, fits, order, count, grids, grid_info, x,
+                                   y, guess);
+            else
+                hipLaunchKernelGGL((guess_ragged_single_kernel<BF_MODEL, 16>), dim3(blocks),
+                                   dim3(256), 0, stream, fits, order, count, grids, grid_info, x,
+                                   y, guess);
+            return hipGetLastError();
+        }
+    }
     const size_t lds = size_t(m_pad + sort_n) * sizeof(float);
     hipLaunchKernelGGL((guess_ragged_kernel<BF_MODEL>), dim3(count), dim3(64), lds, stream, fits,
                        order, count, grids, grid_info, x, y, guess, m_pad, sort_n);

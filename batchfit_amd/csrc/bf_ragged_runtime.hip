@@ -150,6 +150,10 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
             if (top > 256)
                 for (below = 256; 2 * below < top; below *= 2) {
                 }
+            // Single-peak fits up to kGuessSortedMaxPoints take the sorting
+            // guess: no slice spans that bound.
+            if (model != BF_TWO_LORENTZIAN && top > kGuessSortedMaxPoints)
+                below = std::max(below, kGuessSortedMaxPoints);
             size_t end = a;
             uint32_t guessed = 0;
             for (; end < n && fit_table[v[end]].points > below; ++end)

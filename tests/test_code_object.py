@@ -65,11 +65,17 @@ def test_guess_kernels_use_no_private_memory(resources):
     # The guess of the single-peak models: one kernel per model for edge
     # windows up to 32, 64 and 128 values (sorting networks, then the peak),
     # and beyond, edge levels (bisection in 64-lane groups up to 1024, one
-    # wave per fit with the keys read from memory beyond) then the peak.
+    # wave per fit with the keys read from memory beyond, one kernel per
+    # parameter count) then the peak.
     staged = {k: v for k, v in resources.items()
               if "guess_levels_kernel" in k or "guess_levels_stream_kernel" in k
               or "guess_single_kernel" in k or "guess_peak_kernel" in k}
-    assert len(staged) == 15, sorted(staged)
+    assert len(staged) == 16, sorted(staged)
+    # The ragged form of the one-launch kernel (single-peak fits up to
+    # 1,289 points of a ragged batch).
+    ragged = {k: v for k, v in resources.items() if "guess_ragged_single_kernel" in k}
+    assert len(ragged) == 9, sorted(ragged)
+    staged.update(ragged)
     guess.update(staged)
     for name, r in guess.items():
         assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)
