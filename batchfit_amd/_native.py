@@ -22,7 +22,8 @@ MAP_AUTO, MAP_LANE, MAP_GROUP, MAP_LATENCY = 0, 1, 2, 3
 # Every symbol include/batchfit_hip.h declares.
 EXPORTS = (
     "bf_hip_abi_version", "bf_hip_available", "bf_hip_default_config", "bf_hip_fit",
-    "bf_hip_fit_device", "bf_hip_fit_device_noalloc", "bf_hip_guess", "bf_hip_model_eval",
+    "bf_hip_fit_device", "bf_hip_fit_device_noalloc", "bf_hip_guess", "bf_hip_guess_form",
+    "bf_hip_model_eval",
     "bf_hip_plan", "bf_hip_plan_batch", "bf_hip_set_timing",
     "bf_hip_last_timing", "bf_hip_malloc", "bf_hip_free", "bf_hip_memcpy", "bf_hip_synchronize",
     "bf_hip_ragged_create", "bf_hip_ragged_run", "bf_hip_ragged_timing", "bf_hip_ragged_destroy",
@@ -96,6 +97,8 @@ def library() -> ctypes.CDLL:
     lib.bf_hip_fit_device_noalloc.restype = i32
     lib.bf_hip_guess.argtypes = [u32, u32, u32, vp, vp, vp, vp, vp, ctypes.c_char_p, sz]
     lib.bf_hip_guess.restype = i32
+    lib.bf_hip_guess_form.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, ctypes.c_char_p, sz]
+    lib.bf_hip_guess_form.restype = i32
     lib.bf_hip_model_eval.argtypes = [u32, u32, vp, vp, vp, vp, ctypes.c_char_p, sz]
     lib.bf_hip_model_eval.restype = i32
     lib.bf_hip_plan.argtypes = [u32, u32, u32, vp, ctypes.POINTER(u32), ctypes.POINTER(u32),

@@ -39,9 +39,10 @@ __host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
     return n;
 }
 
-// Largest M the device guess takes: the signal and the sort buffer of its
-// edge medians fit in 64 KiB of LDS (12288 + 4096 floats).
-constexpr uint32_t kGuessMaxPoints = 12288;
+// LDS the one-wave guess may take per fit (the signal and the sort buffer of
+// its edge medians, or its suffix minima): M up to 12,288 points. Longer
+// spectra take the wide guess (bf_guess_wide.hpp), which stages nothing.
+constexpr uint32_t kGuessLdsBytes = 65536;
 // Largest M whose edge windows (M / 10 values each) the sorting-network guess
 // of the single-peak models holds: 128 values, eight lanes of 16 slots.
 constexpr uint32_t kGuessSortedMaxPoints = 1289;
@@ -55,6 +56,16 @@ __host__ __device__ inline uint32_t guess_extra_floats(int model, uint32_t m) {
     const uint32_t steps = model == TWO_LORENTZIAN ? (m + 63) / 64 : 0;
     return sort > steps ? sort : steps;
 }
+
+// Whether the one-wave guess of `model` holds M points in its LDS.
+__host__ __device__ inline bool guess_fits_lds(int model, uint32_t m) {
+    return (uint64_t(m) + guess_extra_floats(model, m)) * sizeof(float) <= kGuessLdsBytes;
+}
+
+// Which guess kernel serves a launch: the fastest that applies (AUTO), or,
+// for tests, the one-wave kernel with the signal in LDS (WAVE) or the wide
+// kernel that reads memory (WIDE).
+enum GuessForm : uint32_t { GUESS_AUTO = 0, GUESS_WAVE = 1, GUESS_WIDE = 2 };
 
 enum : int32_t { GS_SUCCESS = 0, GS_INVALID = 1, GS_NO_PEAK = 2, GS_INSUFFICIENT = 3 };
 
@@ -298,7 +309,7 @@ struct GuessPeak {
     bool valid;
 };
 
-__device__ __forceinline__ float smooth_at(const float *s, int m, int i) {
+template <typename S> __device__ __forceinline__ float smooth_at(const S &s, int m, int i) {
     if (i == 0 || i + 1 == m)
         return s[i];
     return 0.25f * s[i - 1] + 0.5f * s[i] + 0.25f * s[i + 1];
@@ -306,7 +317,7 @@ __device__ __forceinline__ float smooth_at(const float *s, int m, int i) {
 
 // Smoothed value at 1 ≤ j < m as smooth_at (the raw value at the last
 // point); j ≥ m gives 0, a value no caller uses.
-__device__ __forceinline__ float smooth_in(const float *s, int m, int j) {
+template <typename S> __device__ __forceinline__ float smooth_in(const S &s, int m, int j) {
     if (j + 1 < m)
         return 0.25f * s[j - 1] + 0.5f * s[j] + 0.25f * s[j + 1];
     return j < m ? s[j] : 0.0f;
@@ -328,7 +339,8 @@ __device__ __forceinline__ float from_lane_above(float v, float last) {
 }
 
 // initial_guess.hpp:61-109 with both half-height walks as wave reductions.
-__device__ GuessPeak measure_peak_dev(const float *x, const float *s, int m, int index, int begin,
+template <typename S>
+__device__ GuessPeak measure_peak_dev(const float *x, const S &s, int m, int index, int begin,
                                       int end, bool mirror, int lane) {
     GuessPeak r{0, 0, 0, 0, 0, false};
     r.amplitude = s[index];
@@ -394,7 +406,7 @@ __device__ GuessPeak measure_peak_dev(const float *x, const float *s, int m, int
     return r;
 }
 
-__device__ __forceinline__ int raw_peak_dev(const float *s, int i) {
+template <typename S> __device__ __forceinline__ int raw_peak_dev(const S &s, int i) {
     int best = i - 1;
     for (int j = i; j <= i + 1; ++j)
         if (s[best] < s[j])
@@ -414,7 +426,8 @@ __device__ __forceinline__ int raw_peak_dev(const float *s, int i) {
 // start); a step that holds such a point decides the side by the nearer of
 // the two kinds. t ≥ 0, so no point is of both kinds, and the points beyond
 // an edge (taken as v = h) are of neither.
-__device__ bool prominent_wave(const float *s, int m, int j, float h, float t, int lane) {
+template <typename S>
+__device__ bool prominent_wave(const S &s, int m, int j, float h, float t, int lane) {
     bool left = false;
     for (int k0 = j - 1; k0 >= 0; k0 -= 64) {
         const int k = k0 - lane;
@@ -445,7 +458,8 @@ __device__ bool prominent_wave(const float *s, int m, int j, float h, float t, i
 // passes), 2 if a point above h or the edge comes first (it fails), 0 if
 // neither lies within reach.
 constexpr int kLocalWalk = 6;
-__device__ __forceinline__ int local_side(const float *s, int m, int j, float h, float t, int dir) {
+template <typename S>
+__device__ __forceinline__ int local_side(const S &s, int m, int j, float h, float t, int dir) {
     for (int d = 1; d <= kLocalWalk; ++d) {
         const int k = j + dir * d;
         if (k < 0 || k >= m)
@@ -459,7 +473,6 @@ __device__ __forceinline__ int local_side(const float *s, int m, int j, float h,
     return 0;
 }
 
-// The guess for fit `fit` of `a`, run by one whole wave.
 // Result of one fit's guess, identical in every lane of the wave.
 struct GuessResult {
     float p[8];
@@ -467,66 +480,21 @@ struct GuessResult {
     int32_t status;
 };
 
-// The guess of one fit by one whole wave (lane = lane in the wave), on the
-// wave's own LDS: s[0..m) holds the fit's raw observations on entry (it is
-// turned into the baseline-removed signal), buf has the sort floats the host
-// reserved (guess_extra_floats), y_bad says whether any observation is
-// non-finite (initial_guess.hpp:131-139).
-template <int MODEL>
-__device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, GridInfo grid,
-                                  bool y_bad, int lane) {
+// The guess from its edge levels on (initial_guess.hpp:170-249), by one whole
+// wave: s[j] is the baseline-removed signal (finite at every point), sfx has
+// one float per 2^sfx_shift 64-point steps of the second-peak search (the
+// two-peak model; ((m − 2) / 64 >> sfx_shift) + 1 of them), res holds the
+// baseline and GS_INVALID on entry.
+template <int MODEL, typename S>
+__device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const float *x, int m,
+                                  const GridInfo &grid, float left_level, float slope, float noise,
+                                  int lane, GuessResult res) {
     constexpr int P = Params<MODEL>::P;
     constexpr bool paired = MODEL == TWO_LORENTZIAN;
-    GuessResult res;
     float *const out = res.p;
-    for (int k = 0; k < 8; ++k)
-        out[k] = qnan_f();
-    float &baseline = res.baseline;
     int32_t &gstatus = res.status;
-    baseline = qnan_f();
-    gstatus = GS_INVALID;
     auto finish = [&]() { return res; };
-    if (m < 5 || !grid.guess_grid_ok || y_bad)
-        return finish();
-    const int count = grid.count;
-    // Edge levels, baseline and noise scale (initial_guess.hpp:142-169).
-    const float left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
-    const float right_level =
-        wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
-    baseline = wave_median(buf, 2 * count, lane, [&](int i) {
-        return i < count ? s[i] : s[m - count + (i - count)];
-    });
-    const float spread = wave_median(buf, 2 * count, lane, [&](int i) {
-        return i < count ? fabsf(s[i] - left_level) : fabsf(s[m - count + (i - count)] - right_level);
-    });
-    const float left_x = grid.left_x, right_x = grid.right_x;
-    const float slope = paired ? (right_level - left_level) / (right_x - left_x) : 0.0f;
-    const float level = paired ? left_level : baseline;
-    if (!finite_f(slope))
-        return finish();
-    const float noise = 1.4826f * spread;
-    int bad = 0;
-    if (paired || __float_as_uint(level) == 0x80000000u) {
-#pragma unroll 8
-        for (int j = lane; j < m; j += 64) {
-            const float v = s[j] - (level + slope * (x[j] - left_x));
-            bad |= !finite_f(v);
-            s[j] = v;
-        }
-    } else {
-        // slope is +0 and x finite, so level + slope·(x − left_x) is level
-        // itself unless level is −0: the grid is not read.
-#pragma unroll 8
-        for (int j = lane; j < m; j += 64) {
-            const float v = s[j] - level;
-            bad |= !finite_f(v);
-            s[j] = v;
-        }
-    }
-    bad = wave_or_i(bad);
-    wave_sync();
-    if (bad || !finite_f(noise))
-        return finish();
+    const float left_x = grid.left_x;
     // Strongest smoothed local maximum (initial_guess.hpp:170-182). Each
     // smoothed value is formed once, in the lane of its point; a point's
     // neighbours come from the adjacent lanes, and across a step's edges
@@ -579,18 +547,18 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
         const float t = 3.0f * noise;
         // A side can pass only if it holds a point with h − v > t, so only if
         // h − (minimum over any range that contains the side) > t. The
-        // minima of the 64-point steps from step c to the end (sfx[c]) and
-        // from the start through the current step (pmin) reject most
-        // candidates without a walk.
-        float *const sfx = buf;
+        // minima from the first 64-point step of a run of 2^sfx_shift steps
+        // to the end (sfx[run]) and from the start through the current step
+        // (pmin) reject most candidates without a walk.
         {
+            const int run_mask = (1 << sfx_shift) - 1;
             float running = __uint_as_float(0x7f800000u);
             for (int c = (m - 2) / 64; c >= 0; --c) {
                 const int j = 1 + 64 * c + lane;
                 running = fminf(running, wave_min_fx(j < m ? smooth_at(s, m, j)
                                                            : __uint_as_float(0x7f800000u)));
-                if (lane == 0)
-                    sfx[c] = running;
+                if (lane == 0 && (c & run_mask) == 0)
+                    sfx[c >> sfx_shift] = running;
             }
             wave_sync();
         }
@@ -609,7 +577,7 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
             cur = next;
             bool cand = false;
             pmin = fminf(pmin, wave_min_fx(j + 1 < m ? v : __uint_as_float(0x7f800000u)));
-            const float right_min = sfx[(base - 1) / 64];
+            const float right_min = sfx[((base - 1) / 64) >> sfx_shift];
             if (j + 1 < m) {
                 cand = v > v_best && v - pmin > t && v - right_min > t && v > left && v >= right;
                 if (cand) {
@@ -684,6 +652,67 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
     }
     gstatus = GS_SUCCESS;
     return finish();
+}
+
+// The guess of one fit by one whole wave (lane = lane in the wave), on the
+// wave's own LDS: s[0..m) holds the fit's raw observations on entry (it is
+// turned into the baseline-removed signal), buf has the sort floats the host
+// reserved (guess_extra_floats), y_bad says whether any observation is
+// non-finite (initial_guess.hpp:131-139).
+template <int MODEL>
+__device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, GridInfo grid,
+                                  bool y_bad, int lane) {
+    constexpr bool paired = MODEL == TWO_LORENTZIAN;
+    GuessResult res;
+    for (int k = 0; k < 8; ++k)
+        res.p[k] = qnan_f();
+    float &baseline = res.baseline;
+    baseline = qnan_f();
+    res.status = GS_INVALID;
+    auto finish = [&]() { return res; };
+    if (m < 5 || !grid.guess_grid_ok || y_bad)
+        return finish();
+    const int count = grid.count;
+    // Edge levels, baseline and noise scale (initial_guess.hpp:142-169).
+    const float left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
+    const float right_level =
+        wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
+    baseline = wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? s[i] : s[m - count + (i - count)];
+    });
+    const float spread = wave_median(buf, 2 * count, lane, [&](int i) {
+        return i < count ? fabsf(s[i] - left_level) : fabsf(s[m - count + (i - count)] - right_level);
+    });
+    const float left_x = grid.left_x, right_x = grid.right_x;
+    const float slope = paired ? (right_level - left_level) / (right_x - left_x) : 0.0f;
+    const float level = paired ? left_level : baseline;
+    if (!finite_f(slope))
+        return finish();
+    const float noise = 1.4826f * spread;
+    int bad = 0;
+    if (paired || __float_as_uint(level) == 0x80000000u) {
+#pragma unroll 8
+        for (int j = lane; j < m; j += 64) {
+            const float v = s[j] - (level + slope * (x[j] - left_x));
+            bad |= !finite_f(v);
+            s[j] = v;
+        }
+    } else {
+        // slope is +0 and x finite, so level + slope·(x − left_x) is level
+        // itself unless level is −0: the grid is not read.
+#pragma unroll 8
+        for (int j = lane; j < m; j += 64) {
+            const float v = s[j] - level;
+            bad |= !finite_f(v);
+            s[j] = v;
+        }
+    }
+    bad = wave_or_i(bad);
+    wave_sync();
+    if (bad || !finite_f(noise))
+        return finish();
+    return guess_tail<MODEL>(static_cast<const float *>(s), buf, 0, x, m, grid, left_level, slope,
+                             noise, lane, res);
 }
 
 // The guess of fit `fit` of `a`, run by one whole wave that owns the dynamic

@@ -343,18 +343,14 @@ __global__ void __launch_bounds__(256) guess_levels_kernel(GuessArgs a, int para
     }
 }
 
-// Stage 1 for edge windows beyond 1024 values (M ≥ 10,250): one wave per fit,
-// the same searches with the window's keys read from memory at every step
-// (the two edges stay in the L2 cache), so M has no limit.
-template <int P> __global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a) {
-    constexpr int param_stride = P;
-    const uint32_t fit = blockIdx.x * 4 + threadIdx.x / 64;
-    const int lane = int(threadIdx.x & 63);
-    const bool live = fit < a.n;
-    const uint32_t f = live ? fit : a.n - 1;
-    const int m = int(a.m);
-    const int count = a.grid->count;
-    const float *y = a.y + size_t(f) * size_t(a.ys_fit);
+// Edge levels of one fit's row y[0..m) by one whole wave, the window keys
+// read from memory at every step of the searches (the two edges stay in the
+// L2 cache), so M has no limit: the median of each edge, of both, and of the
+// absolute deviations from each edge's own median (initial_guess.hpp:142-165).
+struct EdgeLevels {
+    float left, right, both, spread;
+};
+__device__ EdgeLevels stream_levels(const float *y, int m, int count, int lane) {
     const float *right = y + (m - count);
     // mean ∓ σ/2 of a window, as mean_bracket.
     auto bracket = [&](auto value_at, int n, uint32_t &lo, uint32_t &hi) {
@@ -398,9 +394,22 @@ template <int P> __global__ void __launch_bounds__(256) guess_levels_stream_kern
     auto dk = [&](int e) { return float_key(dev_v(e)); };
     const float spread =
         median_around(StreamKeys<decltype(dk)>{dk, 2 * count}, 2 * count, ba, bb).value;
+    return {left_level, right_level, um.value, spread};
+}
+
+// Stage 1 for edge windows beyond 1024 values (M ≥ 10,250): one wave per fit,
+// stream_levels, so M has no limit.
+template <int P> __global__ void __launch_bounds__(256) guess_levels_stream_kernel(GuessArgs a) {
+    constexpr int param_stride = P;
+    const uint32_t fit = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = int(threadIdx.x & 63);
+    const bool live = fit < a.n;
+    const uint32_t f = live ? fit : a.n - 1;
+    const EdgeLevels lv =
+        stream_levels(a.y + size_t(f) * size_t(a.ys_fit), int(a.m), a.grid->count, lane);
     if (live && lane == 0) {
-        a.params[size_t(fit) * param_stride] = um.value;
-        a.params[size_t(fit) * param_stride + 1] = 1.4826f * spread;
+        a.params[size_t(fit) * param_stride] = lv.both;
+        a.params[size_t(fit) * param_stride + 1] = 1.4826f * lv.spread;
     }
 }
 

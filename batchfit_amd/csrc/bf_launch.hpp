@@ -104,7 +104,7 @@ template <int MODEL> hipError_t launch_lm_model(const Plan &plan, const FitArgs 
 template <int MODEL, int G> hipError_t launch_lm_group(const Plan &plan, const FitArgs &args,
                                                        hipStream_t stream);
 template <int MODEL> hipError_t launch_guess_model(const GuessArgs &args, size_t lds_bytes,
-                                                   hipStream_t stream);
+                                                   uint32_t form, hipStream_t stream);
 template <int MODEL> hipError_t launch_eval_model(const EvalArgs &args, hipStream_t stream);
 
 // Lanes per fit and whether the points stream, by (model, M), from the

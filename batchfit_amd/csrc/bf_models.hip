@@ -1,7 +1,7 @@
 // bf_models.hip — per-model dispatch over the (model, group) launchers, the
 // guess kernels and the model-evaluation test kernels.
 #include "bf_launch.hpp"
-#include "bf_guess_group.hpp"
+#include "bf_guess_wide.hpp"
 
 namespace bfhip {
 
@@ -44,10 +44,20 @@ BF_DECLARE_GROUPS(3)
         return hipErrorInvalidValue;                                                           \
     }                                                                                          \
     template <> hipError_t launch_guess_model<M>(const GuessArgs &args, size_t lds_bytes,      \
-                                                 hipStream_t stream) {                         \
+                                                 uint32_t form, hipStream_t stream) {          \
         hipError_t e = hipSuccess;                                                             \
-        if (launch_guess_group<M>(args, stream, e))                                            \
+        if (form == GUESS_AUTO && launch_guess_group<M>(args, stream, e))                      \
             return e;                                                                          \
+        if (form == GUESS_WIDE || (form == GUESS_AUTO && lds_bytes > kGuessLdsBytes)) {        \
+            if (args.ys_pt != 1)                                                               \
+                return hipErrorInvalidValue;                                                   \
+            hipLaunchKernelGGL((guess_wide_kernel<M>), dim3((args.n + kWideWaves - 1) /        \
+                                                            kWideWaves),                       \
+                               dim3(64 * kWideWaves), 0, stream, args);                        \
+            return hipGetLastError();                                                          \
+        }                                                                                      \
+        if (lds_bytes > kGuessLdsBytes)                                                        \
+            return hipErrorInvalidValue;                                                       \
         hipLaunchKernelGGL((guess_kernel<M>), dim3(args.n), dim3(64), lds_bytes, stream, args); \
         return hipGetLastError();                                                              \
     }                                                                                          \

@@ -3,7 +3,7 @@
 // plain FP32 and compensated-mixed numerics).
 #pragma once
 
-#include "bf_guess_group.hpp"
+#include "bf_guess_wide.hpp"
 #include "bf_ragged.hpp"
 
 #include <algorithm>
@@ -43,6 +43,29 @@ __global__ void __launch_bounds__(256) guess_ragged_single_kernel(
     float level, noise;
     sort_levels<S>(g, 0, int(threadIdx.x & 7), level, noise);
     peak_stage<MODEL, 8>(g, 0, level, noise);
+}
+
+// Data-only guess of ragged fits too long for the one-wave guess's LDS: the
+// wide guess, one wave per fit, four per block.
+template <int MODEL>
+__global__ void __launch_bounds__(64 * kWideWaves) guess_ragged_wide_kernel(
+    const bf_fit_desc *fits, const uint32_t *order, uint32_t count, const bf_grid_desc *grids,
+    const GridInfo *grid_info, const float *x, const float *y, float *guess) {
+    constexpr int P = Params<MODEL>::P;
+    __shared__ float sfx[kWideWaves][MODEL == TWO_LORENTZIAN ? kWideSuffix : 1];
+    const int wave = int(threadIdx.x / 64), lane = int(threadIdx.x & 63);
+    const uint32_t task = blockIdx.x * kWideWaves + uint32_t(wave);
+    if (task >= count)
+        return;  // the whole wave
+    const bf_fit_desc d = fits[order[task]];
+    if (d.p0_row != 0xffffffffu)
+        return;
+    const GuessResult g = guess_core_wide<MODEL>(y + d.y_offset, x + grids[d.grid].x_offset,
+                                                 int(d.points), grid_info[d.grid], sfx[wave],
+                                                 lane);
+    if (lane == 0)
+        for (int k = 0; k < P; ++k)
+            guess[size_t(d.out_row) * 8 + k] = g.p[k];
 }
 
 namespace {
@@ -194,6 +217,9 @@ template <> hipError_t prepare_ragged<BF_MODEL>(int cls, bool mixed, bool guess)
     }
     if (e == hipSuccess && guess)
         e = hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&guess_ragged_kernel<BF_MODEL>));
+    if (e == hipSuccess && guess)
+        e = hipFuncGetAttributes(
+            &attr, reinterpret_cast<const void *>(&guess_ragged_wide_kernel<BF_MODEL>));
     if constexpr (BF_MODEL != TWO_LORENTZIAN) {
         if (e == hipSuccess && guess)
             e = hipFuncGetAttributes(
@@ -241,6 +267,12 @@ hipError_t launch_ragged_guess<BF_MODEL>(const bf_fit_desc *fits, const uint32_t
         }
     }
     const size_t lds = size_t(m_pad + sort_n) * sizeof(float);
+    if (lds > kGuessLdsBytes) {  // the slice's longest fit does not fit in LDS
+        hipLaunchKernelGGL((guess_ragged_wide_kernel<BF_MODEL>),
+                           dim3((count + kWideWaves - 1) / kWideWaves), dim3(64 * kWideWaves), 0,
+                           stream, fits, order, count, grids, grid_info, x, y, guess);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((guess_ragged_kernel<BF_MODEL>), dim3(count), dim3(64), lds, stream, fits,
                        order, count, grids, grid_info, x, y, guess, m_pad, sort_n);
     return hipGetLastError();

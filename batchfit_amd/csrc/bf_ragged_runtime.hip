@@ -100,8 +100,6 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
             d.points != grid_table[d.grid].points)
             return "fit " + std::to_string(i) +
                    ": model, grid or config out of range, or M differs from its grid";
-        if (d.p0_row == 0xffffffffu && d.points > kGuessMaxPoints)
-            return "the device initial guess supports M up to 12,288 points; pass p0";
         if (d.out_row == 0xffffffffu)
             return "fit " + std::to_string(i) + ": output row 0xffffffff is out of range";
         out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;

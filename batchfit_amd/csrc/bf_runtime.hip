@@ -21,7 +21,7 @@
 namespace bfhip {
 
 template <int MODEL> hipError_t launch_lm_model(const Plan &, const FitArgs &, hipStream_t);
-template <int MODEL> hipError_t launch_guess_model(const GuessArgs &, size_t, hipStream_t);
+template <int MODEL> hipError_t launch_guess_model(const GuessArgs &, size_t, uint32_t, hipStream_t);
 template <int MODEL> hipError_t launch_eval_model(const EvalArgs &, hipStream_t);
 
 namespace {
@@ -162,12 +162,13 @@ hipError_t launch_lm(uint32_t model, const Plan &plan, const FitArgs &args, hipS
     }
     return hipErrorInvalidValue;
 }
-hipError_t launch_guess(uint32_t model, const GuessArgs &args, size_t lds, hipStream_t s) {
+hipError_t launch_guess(uint32_t model, const GuessArgs &args, size_t lds, hipStream_t s,
+                        uint32_t form = GUESS_AUTO) {
     switch (model) {
-    case 0: return launch_guess_model<0>(args, lds, s);
-    case 1: return launch_guess_model<1>(args, lds, s);
-    case 2: return launch_guess_model<2>(args, lds, s);
-    case 3: return launch_guess_model<3>(args, lds, s);
+    case 0: return launch_guess_model<0>(args, lds, form, s);
+    case 1: return launch_guess_model<1>(args, lds, form, s);
+    case 2: return launch_guess_model<2>(args, lds, form, s);
+    case 3: return launch_guess_model<3>(args, lds, form, s);
     }
     return hipErrorInvalidValue;
 }
@@ -313,13 +314,10 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     const float *init = p0;
     if (!init) {
         const uint32_t sort_n = guess_extra_floats(int(model), m);
+        // The single-peak models' guess streams the row; the two-peak guess
+        // holds the signal in LDS up to kGuessLdsBytes and reads it from
+        // memory beyond (no limit on M either way).
         const size_t lds = size_t(m + sort_n) * sizeof(float);
-        // The single-peak models' guess streams the row (no limit on M); the
-        // two-peak guess holds the signal in 64 KiB of LDS.
-        if (model == BF_TWO_LORENTZIAN && lds > 65536)
-            return fail(err, errlen, BF_INVALID_ARGUMENT,
-                        "the device initial guess of two_lorentzian supports M up to 12,288 "
-                        "points; pass p0");
         float *guess = nullptr;
         if (fixed)
             guess = reinterpret_cast<float *>(fixed + 512);  // every row written by the guess
@@ -606,25 +604,34 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
 
 int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const float *y,
                  float *params, float *baseline, int32_t *gstatus, char *err, size_t errlen) {
+    return bf_hip_guess_form(model, BF_GUESS_AUTO, n, m, x, y, params, baseline, gstatus, err,
+                             errlen);
+}
+
+int bf_hip_guess_form(uint32_t model, uint32_t form, uint32_t n, uint32_t m, const float *x,
+                      const float *y, float *params, float *baseline, int32_t *gstatus, char *err,
+                      size_t errlen) {
     bf_config cfg;
     bf_hip_default_config(&cfg);
     cfg.points = m;
     cfg.fits = n;
     if (const int rc = check_common(model, &cfg, err, errlen))
         return rc;
+    if (form > BF_GUESS_WIDE)
+        return fail(err, errlen, BF_INVALID_ARGUMENT, "unknown guess form");
     if (n == 0)
         return BF_OK;
     if (!x || !y || !params || !baseline || !gstatus)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "every array must be non-null");
+    const uint32_t sort_n = guess_extra_floats(int(model), m);
+    const size_t lds = size_t(m + sort_n) * sizeof(float);
+    if (form == BF_GUESS_WAVE && lds > kGuessLdsBytes)
+        return fail(err, errlen, BF_INVALID_ARGUMENT,
+                    "the one-wave guess holds M up to 12,288 points in LDS");
     Runtime &rt = Runtime::get();
     std::lock_guard<std::mutex> lock(rt.host_calls);
     hipStream_t s = rt.stream;
     const size_t P = size_t(kParams[model]);
-    const uint32_t sort_n = guess_extra_floats(int(model), m);
-    const size_t lds = size_t(m + sort_n) * sizeof(float);
-    if (model == BF_TWO_LORENTZIAN && lds > 65536)
-        return fail(err, errlen, BF_INVALID_ARGUMENT,
-                    "the device initial guess of two_lorentzian supports M up to 12,288 points");
     Scratch scratch(s);
     float *dx, *dy, *dp, *db;
     int32_t *dg;
@@ -640,7 +647,7 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
     hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, s, dx, m, grid, nullptr);
     BF_TRY(hipGetLastError(), "grid prologue launch");
     GuessArgs ga{dx, dy, (long long)m, 1, grid, n, m, m, sort_n, dp, db, dg};
-    BF_TRY(launch_guess(model, ga, lds, s), "initial guess launch");
+    BF_TRY(launch_guess(model, ga, lds, s, form), "initial guess launch");
     BF_TRY(hipMemcpyAsync(params, dp, size_t(n) * P * 4, hipMemcpyDeviceToHost, s), "copy");
     BF_TRY(hipMemcpyAsync(baseline, db, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");
     BF_TRY(hipMemcpyAsync(gstatus, dg, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");

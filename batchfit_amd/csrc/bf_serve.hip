@@ -586,9 +586,6 @@ int bf_serve_submit(bf_server server, uint32_t model, const bf_config *cfg, int 
         return fail(err, errlen, BF_INVALID_ARGUMENT, "device arrays must be 4-byte aligned");
     if (cfg->fits && (!x || !y || !params || !status || !iterations || !chi2))
         return fail(err, errlen, BF_INVALID_ARGUMENT, "x, y and every output must be non-null");
-    if (!p0 && cfg->fits && cfg->points > kGuessMaxPoints)
-        return fail(err, errlen, BF_INVALID_ARGUMENT,
-                    "the device initial guess supports M up to 12,288 points; pass p0");
     Server &s = server->server;
     auto r = std::make_unique<Request>();
     r->model = model;

@@ -290,6 +290,19 @@ int bf_hip_synthesize(uint32_t model, uint32_t n, uint32_t m, const float *d_x,
 int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const float *y,
                  float *params, float *baseline, int32_t *gstatus, char *err, size_t errlen);
 
+/* Guess kernels (every form gives the same bits): */
+enum bf_guess_form {
+    BF_GUESS_AUTO = 0, /* the fastest that applies (bf_hip_guess) */
+    BF_GUESS_WAVE = 1, /* one wave per fit, the signal staged in LDS (M <= 12,288) */
+    BF_GUESS_WIDE = 2  /* one wave per fit, the signal read from memory (any M) */
+};
+/* bf_hip_guess by a chosen kernel, for tests that hold the forms to each
+ * other. No reference counterpart (the reference has one CPU guess,
+ * cpu/initial_guess.hpp:120-249). */
+int bf_hip_guess_form(uint32_t model, uint32_t form, uint32_t n, uint32_t m, const float *x,
+                      const float *y, float *params, float *baseline, int32_t *gstatus, char *err,
+                      size_t errlen);
+
 /* Device model evaluation for parity tests (tests/models_metal.mm:19-60):
  * value[i] and jac[i*P..] of model at (x[i], params[i*8..]) for i < n. */
 int bf_hip_model_eval(uint32_t model, uint32_t n, const float *x, const float *params,

@@ -1,6 +1,6 @@
 """Kernel resources of the built library, read from its gfx950 code objects
-(no device needed): the device guess kernels keep everything in registers
-and LDS (no private segment and no dynamic stack: no stack frames, no
+(no device needed): the device guess kernels keep everything in registers,
+LDS and the rows they read (no private segment and no dynamic stack: no stack frames, no
 out-of-line calls whose arguments live in scratch), and the production LM kernel of the metric
 workload (pseudo-Voigt, 8 lanes, streamed, compensated, FP64 solve, guessed
 starts) neither spills nor uses scratch."""
@@ -76,6 +76,12 @@ def test_guess_kernels_use_no_private_memory(resources):
     ragged = {k: v for k, v in resources.items() if "guess_ragged_single_kernel" in k}
     assert len(ragged) == 9, sorted(ragged)
     staged.update(ragged)
+    # The wide guess (spectra beyond the one-wave guess's LDS), uniform and
+    # ragged, four models.
+    wide = {k: v for k, v in resources.items()
+            if "guess_wide_kernel" in k or "guess_ragged_wide_kernel" in k}
+    assert len(wide) == 8, sorted(wide)
+    staged.update(wide)
     guess.update(staged)
     for name, r in guess.items():
         assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)

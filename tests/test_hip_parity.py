@@ -83,7 +83,12 @@ def test_guess_bitwise_config1(hip, oracle_port):
                               np.ascontiguousarray(b).view(np.uint8))
 
 
-def device_guess(x, y, model):
+GUESS_AUTO, GUESS_WAVE, GUESS_WIDE = 0, 1, 2  # bf_guess_form
+
+
+def device_guess(x, y, model, form=GUESS_AUTO):
+    """bf_hip_guess (form GUESS_AUTO) or bf_hip_guess_form with a chosen kernel:
+    GUESS_WAVE stages each fit's signal in LDS, GUESS_WIDE reads it from memory."""
     from batchfit_amd import _native
     lib = _native.library()
     x = np.ascontiguousarray(x, np.float32)
@@ -93,10 +98,21 @@ def device_guess(x, y, model):
     baseline = np.empty(n, np.float32)
     status = np.empty(n, np.int32)
     err = _native.error_buffer()
-    _native.check(lib.bf_hip_guess(_native.MODELS[model], n, m, x.ctypes.data, y.ctypes.data,
-                                   params.ctypes.data, baseline.ctypes.data, status.ctypes.data,
-                                   err, len(err)), err)
+    if form == GUESS_AUTO:
+        rc = lib.bf_hip_guess(_native.MODELS[model], n, m, x.ctypes.data, y.ctypes.data,
+                              params.ctypes.data, baseline.ctypes.data, status.ctypes.data,
+                              err, len(err))
+    else:
+        rc = lib.bf_hip_guess_form(_native.MODELS[model], form, n, m, x.ctypes.data,
+                                   y.ctypes.data, params.ctypes.data, baseline.ctypes.data,
+                                   status.ctypes.data, err, len(err))
+    _native.check(rc, err)
     return params, baseline, status
+
+
+def bitwise(got, ref):
+    return all(np.array_equal(np.ascontiguousarray(a).view(np.uint8),
+                              np.ascontiguousarray(b).view(np.uint8)) for a, b in zip(got, ref))
 
 
 @pytest.mark.parametrize("model", list(P))
@@ -134,15 +150,10 @@ def test_guess_bitwise_long_windows(hip, oracle_port, model, m):
         assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
 
 
-@pytest.mark.parametrize("m", [700, 2048, 4096])
-def test_two_peak_guess_prominence_walks(hip, oracle_port, m):
-    """The two-peak guess's search for a second prominent maximum on data
-    that make it walk far: one peak only, pure noise, and second peaks from
-    well below to well above three noise scales, on absolute noise. The
-    wave tests each candidate 64 points of a side at a time; the result must
-    equal the oracle's point-by-point walk bit for bit."""
-    rng = np.random.default_rng(m)
-    n = 96
+def prominence_rows(m, n, rng):
+    """Doublet rows that make the second-peak search walk far: one peak only,
+    pure noise, and second peaks from well below to well above three noise
+    scales, on absolute noise."""
     x = np.linspace(-10, 10, m).astype(np.float32)
     a1 = rng.uniform(0.5, 5, n)
     c1 = rng.uniform(-8, 8, n)
@@ -157,14 +168,64 @@ def test_two_peak_guess_prominence_walks(hip, oracle_port, m):
         return a[:, None] / (1 + ((x[None, :] - c[:, None]) / w[:, None]) ** 2)
 
     y = lor(a1, c1, w1) + lor(a2, c2, w2) + rng.normal(0, 0.01, (n, m))
-    y = y.astype(np.float32)
-    got = device_guess(x, y, "two_lorentzian")
+    return x, y.astype(np.float32)
+
+
+@pytest.mark.parametrize("m", [700, 2048, 4096])
+def test_two_peak_guess_prominence_walks(hip, oracle_port, m):
+    """The two-peak guess's search for a second prominent maximum on rows that
+    make it walk far. The wave tests each candidate 64 points of a side at a
+    time; the result must equal the oracle's point-by-point walk bit for bit,
+    with the signal staged in LDS and read from memory alike."""
+    x, y = prominence_rows(m, 96, np.random.default_rng(m))
     ref = oracle_port.guess(x, y, "two_lorentzian", threads=8)
-    for a, b in zip(got, ref):
-        assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
-                              np.ascontiguousarray(b).view(np.uint8)), m
+    assert bitwise(device_guess(x, y, "two_lorentzian"), ref), m
+    assert bitwise(device_guess(x, y, "two_lorentzian", GUESS_WIDE), ref), (m, "wide")
     # Both outcomes of the search occur in the set.
     assert (ref[2] == 0).any() and (ref[2] == 3).any()
+
+
+@pytest.mark.parametrize("model", list(P))
+@pytest.mark.parametrize("m", [5, 64, 257, 1028, 2600, 6000])
+def test_guess_wide_equals_wave_and_oracle(hip, oracle_port, model, m):
+    """The wide guess (one wave per fit, the signal formed from y and x at
+    every read, edge medians searched in memory, coarse suffix minima) against
+    the one-wave guess with the signal staged in LDS and against the oracle,
+    bit for bit, on the comparison set and the peak-scan corner rows
+    (non-finite observations, an overflowing signal, levels of −0, flat
+    tops); on a quantised copy equal in value (the sign of a zero median is
+    not part of the contract)."""
+    x, y, _, _, _ = datasets.comparison_set(model, 40, m)
+    xc, yc = peak_scan_rows(m, np.random.default_rng(m))
+    assert np.array_equal(x.astype(np.float32), xc)
+    rows = np.vstack([y.astype(np.float32), yc])
+    ref = oracle_port.guess(xc, rows, model, threads=8)
+    wide = device_guess(xc, rows, model, GUESS_WIDE)
+    assert bitwise(wide, ref), (model, m)
+    assert bitwise(device_guess(xc, rows, model, GUESS_WAVE), ref), (model, m, "wave")
+    with np.errstate(over="ignore"):  # the overflow rows quantise to ±inf
+        q = (np.round(rows * 50) / 50).astype(np.float32)
+    ref = oracle_port.guess(xc, q, model, threads=8)
+    for a, b in zip(device_guess(xc, q, model, GUESS_WIDE), ref):
+        assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
+
+
+@pytest.mark.parametrize("m", [12289, 16384, 70000])
+def test_two_peak_guess_long_spectra(hip, oracle_port, m):
+    """Doublets beyond the one-wave guess's LDS take the wide guess: bit for
+    bit the oracle's guess (initial_guess.hpp keeps its scratch on the heap,
+    :143), at 70,000 points with suffix minima over pairs of 64-point steps;
+    fits from it within the FP32 gate."""
+    x, y, _, bounds, _ = datasets.comparison_set("two_lorentzian", 6, m)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    xw, yw = prominence_rows(m, 12, np.random.default_rng(m))
+    for xs, ys in ((x, y), (xw, yw)):
+        ref = oracle_port.guess(xs, ys, "two_lorentzian", threads=8)
+        assert bitwise(device_guess(xs, ys, "two_lorentzian"), ref), m
+    assert np.all(oracle_port.guess(x, y, "two_lorentzian", threads=8)[2] == 0)
+    fit = hip.fit(x, y, "two_lorentzian", None, bounds, "hip")
+    assert_fp32_gate(fit, oracle_port.fit(x, y, "two_lorentzian", None, bounds, threads=8),
+                     f"two_lorentzian M={m} guessed")
 
 
 def test_guess_bitwise_metric_and_doublets(hip, oracle_port):
@@ -180,6 +241,7 @@ def test_guess_bitwise_metric_and_doublets(hip, oracle_port):
     for a, b in zip(got, ref):
         assert np.array_equal(np.ascontiguousarray(a).view(np.uint8),
                               np.ascontiguousarray(b).view(np.uint8))
+    assert bitwise(device_guess(x3, y3, "two_lorentzian", GUESS_WIDE), ref)
 
 
 @pytest.mark.parametrize("model", list(P))
@@ -553,11 +615,16 @@ def test_guess_long_spectra(hip, oracle_port, model, m):
                      f"{model} M={m} guessed")
 
 
-def test_two_peak_guess_length_limit(hip):
-    """The two-peak guess holds the signal in LDS: M up to 12,288 points."""
+def test_guess_form_errors(hip):
+    """The one-wave form holds at most 12,288 points in LDS; unknown forms are
+    argument errors."""
+    from batchfit_amd import _native
     x = np.linspace(-5, 5, 12289).astype(np.float32)
+    y = np.ones((2, x.size), np.float32)
     with pytest.raises(ValueError, match="12,288"):
-        hip.fit(x, np.ones((2, x.size), np.float32), "two_lorentzian", None, None, "hip")
+        device_guess(x, y, "two_lorentzian", GUESS_WAVE)
+    with pytest.raises(ValueError, match="unknown guess form"):
+        device_guess(x, y, "two_lorentzian", 3)
 
 
 def peak_scan_rows(m, rng):

@@ -64,6 +64,24 @@ def test_ragged_equals_uniform_path(hip):
                 assert same(u, v), (model, m)
 
 
+def test_ragged_long_spectra_from_guess(hip):
+    """Guessed fits beyond the one-wave guess's LDS (M > 12,288 points) take
+    the wide guess in a ragged batch: the same bytes as the uniform entry
+    (whose single-peak guess is the streamed group form), beside short fits."""
+    from batchfit_amd.ragged import fit_ragged
+    batches = []
+    for model, m, n in (("two_lorentzian", 16384, 5), ("lorentzian", 20000, 4),
+                        ("gaussian", 13000, 3), ("pseudo_voigt", 300, 30)):
+        x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
+        batches.append((x.astype(np.float32), y.astype(np.float32), model, None, bounds))
+    together = fit_ragged(batches)
+    for (x, y, model, p0, bounds), got in zip(batches, together):
+        ref = hip.fit(x, y, model, None, bounds, "hip")
+        for a, b in zip(got, ref):
+            assert same(a, b), (model, x.size)
+        assert np.all(ref.status == 0), (model, x.size)
+
+
 @pytest.mark.parametrize("model,m", [("pseudo_voigt", 900), ("two_lorentzian", 700),
                                      ("gaussian", 3000)])
 def test_ragged_equals_uniform_large_streamed_class(hip, model, m):

@@ -33,6 +33,19 @@ def requests():
     return out
 
 
+def test_served_long_spectra_from_guess(hip):
+    """A request of doublets longer than the one-wave guess's LDS (M = 16,384)
+    without p0 is served from the wide guess: the uniform entry's bytes."""
+    from batchfit_amd.serve import Server
+    x, y, _, bounds, _ = datasets.comparison_set("two_lorentzian", 5, 16384)
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    with Server(slots=2) as srv:
+        got = srv.submit(x, y, "two_lorentzian", None, bounds).wait(timeout=60)
+    ref = hip.fit(x, y, "two_lorentzian", None, bounds, "hip")
+    for a, b in zip(got, ref):
+        assert same(a, b)
+
+
 def test_served_results_equal_uniform_fits(hip):
     from batchfit_amd.serve import Server
     batch = requests()
