@@ -58,6 +58,15 @@ __device__ __forceinline__ float qnan_f() { return __uint_as_float(0x7fc00000u);
 
 template <int MODEL> struct ModelConsts;
 
+// Factor that turns the eval() Jacobian entry k into the model's derivative
+// (1 unless a model evaluates a column at an exact power-of-two scale).
+template <int MODEL> __host__ __device__ constexpr float jac_scale(int k) {
+    if constexpr (MODEL == PSEUDO_VOIGT)
+        return k == 2 ? 0.5f : 1.0f;
+    else
+        return 1.0f;
+}
+
 template <> struct ModelConsts<GAUSSIAN> {
     float a, c, iw, aiw;
     __device__ __forceinline__ void set(const float *p) {
@@ -105,35 +114,42 @@ template <> struct ModelConsts<LORENTZIAN> {
     }
 };
 
+// In terms of u = 2z = 2(x − c)/w: exp(−4 ln2 z²) = exp2(−u²), the sign a
+// source modifier of v_exp_f32, and 1 + 4z² = 1 + u². Scaling by 2 is exact,
+// so u, u², the value and the first, second and fourth Jacobian entries have
+// the bits of the same expressions in z (with 8a/w halved to 4a/w), and the
+// width entry j1·u is exactly twice j1·z: jac_scale undoes that after the
+// sums (fused_pass, information_pass) and in model_eval_kernel.
 template <> struct ModelConsts<PSEUDO_VOIGT> {
-    float a, c, iw, eta, gmix, k;
+    float a, c, iw2, eta, gmix, kh;
     __device__ __forceinline__ void set(const float *p) {
         a = p[0];
         c = p[1];
-        iw = 1.0f / p[2];
+        const float iw = 1.0f / p[2];
+        iw2 = 2.0f * iw;
         eta = p[3];
         gmix = 1.0f - p[3];
-        k = 8.0f * a * iw;
+        kh = 4.0f * a * iw;
     }
     __device__ __forceinline__ float eval(float x, float *jac) const {
-        const float z = (x - c) * iw;
-        const float z2 = z * z;
-        const float g = __builtin_amdgcn_exp2f(-4.0f * z2);
-        const float l = __builtin_amdgcn_rcpf(__builtin_fmaf(4.0f, z2, 1.0f));
+        const float u = (x - c) * iw2;
+        const float u2 = u * u;
+        const float g = __builtin_amdgcn_exp2f(-u2);
+        const float l = __builtin_amdgcn_rcpf(1.0f + u2);
         const float shape = __builtin_fmaf(gmix, g, eta * l);
-        const float scale = k * __builtin_fmaf(gmix * 0.69314718055994530942f, g, eta * l * l);
-        const float j1 = scale * z;
+        const float scale = kh * __builtin_fmaf(gmix * 0.69314718055994530942f, g, eta * l * l);
+        const float j1 = scale * u;
         jac[0] = shape;
         jac[1] = j1;
-        jac[2] = j1 * z;
+        jac[2] = j1 * u;
         jac[3] = a * (l - g);
         return a * shape;
     }
     __device__ __forceinline__ float value(float x) const {
-        const float z = (x - c) * iw;
-        const float z2 = z * z;
-        const float g = __builtin_amdgcn_exp2f(-4.0f * z2);
-        const float l = __builtin_amdgcn_rcpf(__builtin_fmaf(4.0f, z2, 1.0f));
+        const float u = (x - c) * iw2;
+        const float u2 = u * u;
+        const float g = __builtin_amdgcn_exp2f(-u2);
+        const float l = __builtin_amdgcn_rcpf(1.0f + u2);
         return a * __builtin_fmaf(gmix, g, eta * l);
     }
 };

@@ -40,7 +40,7 @@ template <int MODEL> __global__ void __launch_bounds__(256) model_eval_kernel(Ev
     float jac[P];
     a.value[i] = mc.eval(a.x[i], jac);
     for (int k = 0; k < P; ++k)
-        a.jac[size_t(i) * P + k] = jac[k];
+        a.jac[size_t(i) * P + k] = jac[k] * jac_scale<MODEL>(k);
 }
 
 // Shared-grid facts (cpu/solver.hpp:58-69, batch.hpp:236-240,

@@ -27,6 +27,8 @@
 
 #include "bf_guess.hpp"
 
+#include <type_traits>
+
 // Contraction only within one source expression, so that a fit's rounding
 // does not depend on the code around the point loop (register tile or
 // streamed, uniform or ragged launch).
@@ -74,6 +76,19 @@ template <typename S> __device__ __forceinline__ S sabs(S v) { return v < S(0) ?
 
 // Packed lower-triangle index of (k, l), l <= k.
 __host__ __device__ constexpr int tri(int k, int l) { return k * (k + 1) / 2 + l; }
+
+// Factor that turns pass term t (packed JᵀJ, Jᵀr, χ²) summed from eval()
+// Jacobians into the model's own term (jac_scale of its entries).
+template <int MODEL> __host__ __device__ constexpr float term_scale(int t) {
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    if (t < TRI) {
+        int k = 0;
+        while ((k + 1) * (k + 2) / 2 <= t)
+            ++k;
+        return jac_scale<MODEL>(k) * jac_scale<MODEL>(t - k * (k + 1) / 2);
+    }
+    return t < TRI + P ? jac_scale<MODEL>(t - TRI) : 1.0f;
+}
 
 // Checked, unpivoted Cholesky solve on a packed lower triangle
 // (cpu/solver.hpp:99-143). Returns false on a non-positive pivot, a
@@ -146,11 +161,25 @@ __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
 // two-peak model), then the block partial is added into a Kahan pair (sum,
 // carry) (numerics.hpp:12-18), so the compensation costs 4 VALU per term per
 // block instead of per point. A lane's partial is (sum − carry).
+//
+// Flush kinds: FlushClear adds the block into the pair and clears the block
+// partial; FlushKeep adds it and leaves the partial for the next block's
+// first product to overwrite (add_prod with std::true_type: b = x·y, the same
+// value as fma(x, y, +0) except that a −0 product stays −0, which a later
+// product or the pair's additions cannot tell from +0, since the pair starts
+// at +0); ClearOnly clears the partial.
+struct FlushClear {};
+struct FlushKeep {};
+struct ClearOnly {};
 template <bool COMP> struct Acc;
 template <> struct Acc<false> {
     float s;
     __device__ __forceinline__ void zero() { s = 0.0f; }
-    __device__ __forceinline__ void add_prod(float a, float b) { s = __builtin_fmaf(a, b, s); }
+    template <bool FIRST>
+    __device__ __forceinline__ void add_prod(float a, float b, std::bool_constant<FIRST> = {}) {
+        s = __builtin_fmaf(a, b, s);
+    }
+    template <typename Kind> __device__ __forceinline__ void flush(Kind) {}
     __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ float hi() const { return s; }
     __device__ __forceinline__ float lo() const { return 0.0f; }
@@ -158,15 +187,26 @@ template <> struct Acc<false> {
 template <> struct Acc<true> {
     float s, c, b;
     __device__ __forceinline__ void zero() { s = c = b = 0.0f; }
-    __device__ __forceinline__ void add_prod(float x, float y) { b = __builtin_fmaf(x, y, b); }
-    __device__ __forceinline__ void flush() {
+    template <bool FIRST>
+    __device__ __forceinline__ void add_prod(float x, float y, std::bool_constant<FIRST> = {}) {
+        if constexpr (FIRST)
+            b = x * y;
+        else
+            b = __builtin_fmaf(x, y, b);
+    }
+    __device__ __forceinline__ void add_pair() {
 #pragma clang fp contract(off)
         const float adjusted = b - c;
         const float next = s + adjusted;
         c = (next - s) - adjusted;
         s = next;
+    }
+    __device__ __forceinline__ void flush(FlushClear = {}) {
+        add_pair();
         b = 0.0f;
     }
+    __device__ __forceinline__ void flush(FlushKeep) { add_pair(); }
+    __device__ __forceinline__ void flush(ClearOnly) { b = 0.0f; }
     __device__ __forceinline__ float hi() const { return s; }
     __device__ __forceinline__ float lo() const { return c; }
 };
@@ -206,6 +246,15 @@ template <int G, int R, int CHUNK = 16> struct Points {
     const float *x, *y;
     long long ys_pt;
     int gl, m;
+    // Observation j of the fit. Group mappings read fit-major rows (ys_pt is
+    // 1, so a lane's loads take immediate offsets); only the lane-per-fit
+    // form may read a point-major copy.
+    __device__ __forceinline__ float obs(int j) const {
+        if constexpr (G > 1)
+            return y[j];
+        else
+            return y[j * ys_pt];
+    }
     __device__ __forceinline__ void load(const float *x_, const float *yrow, long long ys_pt_,
                                          int gl_, int m_) {
         x = x_;
@@ -217,9 +266,37 @@ template <int G, int R, int CHUNK = 16> struct Points {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = gl + G * r;
-                yv[r] = j < m ? y[j * ys_pt] : 0.0f;
+                yv[r] = j < m ? obs(j) : 0.0f;
             }
         }
+    }
+    // f(x, y) or, where f takes it, f(x, y, first): first is
+    // std::bool_constant<true> for the first point of a whole block (every
+    // lane of the group holds every point of it), after a FlushKeep.
+    template <typename F, bool FIRST>
+    __device__ __forceinline__ static void point(F &f, float xv, float yv,
+                                                 std::bool_constant<FIRST>, bool first) {
+        if constexpr (std::is_invocable_v<F &, float, float, std::true_type>) {
+            if constexpr (FIRST) {
+                if (first)
+                    f(xv, yv, std::true_type{});
+                else
+                    f(xv, yv, std::false_type{});
+            } else {
+                f(xv, yv, std::false_type{});
+            }
+        } else {
+            f(xv, yv);
+        }
+    }
+    // flush() or, where it takes one, flush(kind); a flush() with no kind
+    // runs at FlushKeep (it holds no partials) and not at ClearOnly.
+    template <typename Flush, typename Kind>
+    __device__ __forceinline__ static void flush_as(Flush &flush, Kind kind) {
+        if constexpr (std::is_invocable_v<Flush &, Kind>)
+            flush(kind);
+        else if constexpr (!std::is_same_v<Kind, ClearOnly>)
+            flush();
     }
     // f(x, y) for every point this lane owns, then flush() after every block
     // of kBlock points. Slots below m/G are full for every lane of the group
@@ -233,9 +310,9 @@ template <int G, int R, int CHUNK = 16> struct Points {
                 // Every slot full: straight-line code with no joins.
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    f(x[gl + G * r], yv[r]);
+                    point(f, x[gl + G * r], yv[r], std::bool_constant<false>{}, false);
                     if (r % kBlock == kBlock - 1 || r == R - 1)
-                        flush();
+                        flush_as(flush, FlushClear{});
                 }
                 return;
             }
@@ -245,11 +322,11 @@ template <int G, int R, int CHUNK = 16> struct Points {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (r < full)
-                    f(x[gl + G * r], yv[r]);
+                    point(f, x[gl + G * r], yv[r], std::bool_constant<false>{}, false);
                 else if (r == full && gl + G * r < m)
-                    f(x[gl + G * r], yv[r]);
+                    point(f, x[gl + G * r], yv[r], std::bool_constant<false>{}, false);
                 if ((r % kBlock == kBlock - 1 || r == R - 1) && G * (r - r % kBlock) < m)
-                    flush();
+                    flush_as(flush, FlushClear{});
             }
         } else {
             int j = gl;
@@ -259,40 +336,59 @@ template <int G, int R, int CHUNK = 16> struct Points {
                 // issued before the first point is evaluated, so one memory
                 // latency is paid per chunk instead of per point. Points and
                 // flushes come in the same order as in the per-point loop.
-                static_assert(kChunk % kBlock == 0 || kBlock % kChunk == 0,
-                              "blocks and chunks nest");
+                static_assert(kBlock % kChunk == 0, "a block holds whole chunks");
+                // Whole blocks first: the block's first point starts its
+                // partials and the flush keeps them, so they are cleared once
+                // after the loop instead of at every flush.
+                for (; j + G * (kBlock - 1) < m; j += G * kBlock) {
+#pragma unroll
+                    for (int h = 0; h < kBlock / kChunk; ++h) {
+                        float xc[kChunk], yc[kChunk];
+#pragma unroll
+                        for (int c = 0; c < kChunk; ++c) {
+                            xc[c] = x[j + G * (h * kChunk + c)];
+                            yc[c] = obs(j + G * (h * kChunk + c));
+                        }
+                        point(f, xc[0], yc[0], std::bool_constant<true>{}, h == 0);
+#pragma unroll
+                        for (int c = 1; c < kChunk; ++c)
+                            point(f, xc[c], yc[c], std::bool_constant<false>{}, false);
+                    }
+                    flush_as(flush, FlushKeep{});
+                }
+                flush_as(flush, ClearOnly{});
                 for (; j + G * (kChunk - 1) < m; j += G * kChunk) {
                     float xc[kChunk], yc[kChunk];
 #pragma unroll
                     for (int c = 0; c < kChunk; ++c) {
                         xc[c] = x[j + G * c];
-                        yc[c] = y[(j + G * c) * ys_pt];
+                        yc[c] = obs(j + G * c);
                     }
 #pragma unroll
                     for (int c = 0; c < kChunk; ++c) {
-                        f(xc[c], yc[c]);
+                        point(f, xc[c], yc[c], std::bool_constant<false>{}, false);
                         if constexpr (kChunk >= kBlock)
                             if (c % kBlock == kBlock - 1)
-                                flush();
+                                flush_as(flush, FlushClear{});
                     }
                     if constexpr (kBlock > kChunk) {
                         k += kChunk;
                         if (k == kBlock) {
-                            flush();
+                            flush_as(flush, FlushClear{});
                             k = 0;
                         }
                     }
                 }
             }
             for (; j < m; j += G) {
-                f(x[j], y[j * ys_pt]);
+                point(f, x[j], obs(j), std::bool_constant<false>{}, false);
                 if (++k == kBlock) {
-                    flush();
+                    flush_as(flush, FlushClear{});
                     k = 0;
                 }
             }
             if (k)
-                flush();
+                flush_as(flush, FlushClear{});
             if constexpr (G > 1) {
                 // The register tile flushes every block that holds a point of
                 // the group, so a lane whose points end one block early
@@ -300,7 +396,7 @@ template <int G, int R, int CHUNK = 16> struct Points {
                 // only, whether its points stream or sit in registers.
                 const int mine = (m - gl + G - 1) / G;
                 if ((mine + kBlock - 1) / kBlock < (m + kBlock * G - 1) / (kBlock * G))
-                    flush();
+                    flush_as(flush, FlushClear{});
             }
         }
     }
@@ -324,7 +420,7 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         acc[t].zero();
     // With yrange: also the group's smallest and largest observation.
     float ylo = __uint_as_float(0x7f800000u), yhi = -__uint_as_float(0x7f800000u);
-    pts.each([&](float xj, float yj) {
+    pts.each([&](float xj, float yj, auto first) {
         if (yrange) {
             ylo = fminf(ylo, yj);
             yhi = fmaxf(yhi, yj);
@@ -336,14 +432,14 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         for (int k = 0; k < P; ++k) {
 #pragma unroll
             for (int l = 0; l <= k; ++l)
-                acc[tri(k, l)].add_prod(jac[k], jac[l]);
-            acc[TRI + k].add_prod(jac[k], r);
+                acc[tri(k, l)].add_prod(jac[k], jac[l], first);
+            acc[TRI + k].add_prod(jac[k], r, first);
         }
-        acc[TRI + P].add_prod(r, r);
-    }, [&] {
+        acc[TRI + P].add_prod(r, r, first);
+    }, [&](auto kind) {
 #pragma unroll
         for (int t = 0; t < T; ++t)
-            acc[t].flush();
+            acc[t].flush(kind);
     });
     if (yrange) {
         if constexpr (G > 1) {
@@ -364,14 +460,14 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         group_allreduce_d<G, T>(v, pts.gl);
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            terms[t] = v[t];
+            terms[t] = v[t] * double(term_scale<MODEL>(t));
             if (t < TRI + P)
                 lin_ok = lin_ok && finite_s(terms[t]);
         }
     } else {
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            terms[t] = reduce_term<G, S, COMP>(acc[t]);
+            terms[t] = reduce_term<G, S, COMP>(acc[t]) * term_scale<MODEL>(t);
             if (t < TRI + P)
                 lin_ok = lin_ok && finite_s(terms[t]);
         }
@@ -392,7 +488,7 @@ __device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk
     for (int t = 0; t < TRI; ++t)
         acc[t].zero();
     int bad = 0;
-    pts.each([&](float xj, float) {
+    pts.each([&](float xj, float, auto first) {
         float jac[P];
         (void)mc.eval(xj, jac);
 #pragma unroll
@@ -400,12 +496,12 @@ __device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk
             bad |= !finite_f(jac[k]);
 #pragma unroll
             for (int l = 0; l <= k; ++l)
-                acc[tri(k, l)].add_prod(jac[k], jac[l]);
+                acc[tri(k, l)].add_prod(jac[k], jac[l], first);
         }
-    }, [&] {
+    }, [&](auto kind) {
 #pragma unroll
         for (int t = 0; t < TRI; ++t)
-            acc[t].flush();
+            acc[t].flush(kind);
     });
     if constexpr (G > 1)
         bad = group_or<G>(bad);
@@ -418,11 +514,11 @@ __device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk
         group_allreduce_d<G, TRI>(v, pts.gl);
 #pragma unroll
         for (int t = 0; t < TRI; ++t)
-            info[t] = v[t];
+            info[t] = v[t] * double(term_scale<MODEL>(t));
     } else {
 #pragma unroll
         for (int t = 0; t < TRI; ++t)
-            info[t] = reduce_term<G, S, true>(acc[t]);
+            info[t] = reduce_term<G, S, true>(acc[t]) * term_scale<MODEL>(t);
     }
 }
 

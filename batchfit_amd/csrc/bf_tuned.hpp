@@ -17,9 +17,9 @@ struct TunedRange {
 constexpr int kTunedRanges = 3;
 constexpr TunedRange kTuned[4][kTunedRanges] = {
     {{1280, 4, 1}, {0xffffffffu, 16, 1}, {0xffffffffu, 16, 1}},  // gaussian
-    {{1280, 4, 1}, {0xffffffffu, 16, 1}, {0xffffffffu, 16, 1}},  // lorentzian
+    {{1792, 4, 1}, {0xffffffffu, 16, 1}, {0xffffffffu, 16, 1}},  // lorentzian
     {{448, 4, 1}, {1792, 8, 1}, {0xffffffffu, 16, 1}},  // pseudo_voigt
-    {{112, 4, 1}, {640, 8, 1}, {0xffffffffu, 16, 1}},  // two_lorentzian
+    {{160, 4, 1}, {448, 8, 1}, {0xffffffffu, 16, 1}},  // two_lorentzian
 };
 
 }  // namespace bfhip
