@@ -308,6 +308,25 @@ struct RaggedRun {
     std::vector<GuessSlice> guess_slices;  // octaves that hold a guessed fit
 };
 
+// Relative work per point of a fit by model (points × parameter-weighted
+// passes: a P=8 fit costs about three times a P=3 one per point), for run
+// ordering and branch balance.
+constexpr double kRaggedWeight[4] = {1.0, 1.0, 1.5, 3.5};
+
+// Lower end of the guess slice that holds a run's fits of at most `top`
+// points, longest first: the octave (2^k, 2^(k+1)] of M (all M <= 256
+// together), and single-peak fits of at most kGuessSortedMaxPoints apart
+// from longer ones (they take the sorting guess).
+inline uint32_t guess_slice_floor(uint32_t model, uint32_t top) {
+    uint32_t below = 0;  // the octave is (below, 2·below]
+    if (top > 256)
+        for (below = 256; 2 * below < top; below *= 2) {
+        }
+    if (model != TWO_LORENTZIAN && top > kGuessSortedMaxPoints)
+        below = below > kGuessSortedMaxPoints ? below : kGuessSortedMaxPoints;
+    return below;
+}
+
 // Host view of a ragged batch: validated descriptors bucketed by (model,
 // class), each bucket longest-first and, when asked, cut into runs of at
 // most about an eighth of the batch's work.

@@ -119,12 +119,9 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
                 return "output row " + std::to_string(rows[i]) + " is named by more than one fit";
     }
     out.order.reserve(fits);
-    // Relative work: points × parameters-weighted passes (a P=8 fit costs
-    // about three times a P=3 one per point).
-    static constexpr double kWeight[4] = {1.0, 1.0, 1.5, 3.5};
     double total = 0.0;
     for (uint32_t i = 0; i < fits; ++i)
-        total += double(fit_table[i].points) * kWeight[fit_table[i].model];
+        total += double(fit_table[i].points) * kRaggedWeight[fit_table[i].model];
     // A bucket heavier than an eighth of the batch becomes several runs over
     // consecutive slices of its longest-first order, so that the branches of
     // the graph can balance (a fit's result does not depend on its run).
@@ -144,14 +141,7 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         // v is longest first, so each octave of M is a contiguous slice.
         for (size_t a = 0; a < n;) {
             const uint32_t top = fit_table[v[a]].points;
-            uint32_t below = 0;  // the octave is (below, 2·below]
-            if (top > 256)
-                for (below = 256; 2 * below < top; below *= 2) {
-                }
-            // Single-peak fits up to kGuessSortedMaxPoints take the sorting
-            // guess: no slice spans that bound.
-            if (model != BF_TWO_LORENTZIAN && top > kGuessSortedMaxPoints)
-                below = std::max(below, kGuessSortedMaxPoints);
+            const uint32_t below = guess_slice_floor(model, top);
             size_t end = a;
             uint32_t guessed = 0;
             for (; end < n && fit_table[v[end]].points > below; ++end)
@@ -160,7 +150,7 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
                 r.guess_slices.push_back({uint32_t(a), uint32_t(end - a), top});
             a = end;
         }
-        r.cost = points * kWeight[model];
+        r.cost = points * kRaggedWeight[model];
         out.order.insert(out.order.end(), v, v + n);
         out.runs.push_back(std::move(r));
     };
@@ -174,7 +164,7 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         size_t a = 0;
         double acc = 0.0;
         for (size_t k = 0; k < v.size(); ++k) {
-            acc += double(fit_table[v[k]].points) * kWeight[model];
+            acc += double(fit_table[v[k]].points) * kRaggedWeight[model];
             if (acc >= piece_cost && k + 1 < v.size()) {
                 add_run(model, cls, v.data() + a, k + 1 - a);
                 a = k + 1;

@@ -20,11 +20,14 @@
 #include "bf_host.hpp"
 #include "bf_ragged.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <cstring>
 #include <memory>
+#include <numeric>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -158,12 +161,95 @@ uint64_t request_bytes(const Request &r) {
 
 }  // namespace
 
+// One request of a batch as the device sees it: where its fits' rows,
+// observations, work order and p0 rows lie, and (device requests) where its
+// outputs go.
+struct ServeReq {
+    uint64_t y_off;      // first observation, in floats from the batch's y base
+    uint32_t m, model;
+    uint32_t first_row;  // batch row of its first fit (rows of a batch increase with the request)
+    uint32_t n;
+    uint32_t order_pos;  // position of its first fit in the work order
+    uint32_t p0_row;     // batch p0 row of its first fit, or 0xffffffff (device guess)
+    uint32_t P, pad;
+    const float *p0_dev;  // a device request's own p0 [n, P], gathered into rows of 8
+    float *params, *chi2;  // a device request's outputs (null for host requests)
+    int32_t *status;
+    uint32_t *iterations;
+};
+
+// The request that holds batch row i: the last one whose first row is at or
+// below i (every request of a batch has at least one fit).
+__device__ __forceinline__ uint32_t request_of(const ServeReq *req, uint32_t nreq, uint32_t i) {
+    uint32_t lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (req[mid].first_row <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Fit descriptors, the work order and device requests' p0 rows of a batch,
+// one thread per row, from the request table.
+__global__ void __launch_bounds__(256) serve_expand_kernel(const ServeReq *req, uint32_t nreq,
+                                                           uint32_t rows, bf_fit_desc *fits,
+                                                           uint32_t *order, float *p0_rows) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows)
+        return;
+    const uint32_t q = request_of(req, nreq, i);
+    const ServeReq &r = req[q];
+    const uint32_t k = i - r.first_row;
+    bf_fit_desc d;
+    d.y_offset = r.y_off + uint64_t(k) * r.m;
+    d.points = r.m;
+    d.grid = q;
+    d.model = r.model;
+    d.config = q;
+    d.p0_row = r.p0_row == 0xffffffffu ? 0xffffffffu : r.p0_row + k;
+    d.out_row = i;
+    fits[i] = d;
+    order[r.order_pos + k] = i;
+    if (r.p0_dev)
+        for (uint32_t p = 0; p < r.P; ++p)
+            p0_rows[size_t(d.p0_row) * 8 + p] = r.p0_dev[size_t(k) * r.P + p];
+}
+
+// Outputs of the device requests of a batch, one thread per row, from the
+// batch's rows (params in rows of 8).
+__global__ void __launch_bounds__(256) serve_scatter_kernel(const ServeReq *req, uint32_t nreq,
+                                                            uint32_t rows, const float *params,
+                                                            const int32_t *status,
+                                                            const uint32_t *iters,
+                                                            const float *chi2) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows)
+        return;
+    const ServeReq &r = req[request_of(req, nreq, i)];
+    if (!r.params)
+        return;
+    const uint32_t k = i - r.first_row;
+    for (uint32_t p = 0; p < r.P; ++p)
+        r.params[size_t(k) * r.P + p] = params[size_t(i) * 8 + p];
+    r.status[k] = status[i];
+    r.iterations[k] = iters[i];
+    r.chi2[k] = chi2[i];
+}
+
 // Pack and enqueue one admitted batch on its slot (called without the lock).
+// The host works per request: it stages host requests' arrays and the small
+// tables (one upload), buckets the requests by (model, mapping class) and
+// orders each bucket longest first, which gives the runs and their guess
+// slices (a request's fits share M, so it never straddles a slice); the
+// device expands the requests into fit descriptors and the work order, and
+// scatters device requests' outputs, in one kernel each.
 void Server::launch(Slot &slot) {
     auto &batch = slot.batch;
     uint64_t xs = 0, ys = 0, p0s = 0, rows = 0;
-    bool unc = false;
-    bool any_host = false;
+    bool unc = false, any_host = false, any_device = false;
     for (Request *r : batch) {
         r->out_row = rows;
         if (!r->device) {
@@ -172,6 +258,8 @@ void Server::launch(Slot &slot) {
             xs += r->m;
             ys += uint64_t(r->n) * r->m;
             any_host = true;
+        } else {
+            any_device = true;
         }
         if (r->p0) {
             r->p0_row = p0s;
@@ -183,35 +271,40 @@ void Server::launch(Slot &slot) {
     slot.rows = uint32_t(rows);
     slot.uncertainties = unc;
     const uint32_t nreq = uint32_t(batch.size());
-    // Device layout: x | y | p0 | fits | grids | configs | info | order | cursors | guess | outputs
+    // Both arenas begin with the uploaded prefix x | y | p0 | grids | configs
+    // | requests (host requests' arrays staged, device requests' p0 rows
+    // gathered on the device afterwards); the device goes on with its work
+    // tables and the outputs, the pinned arena with the same outputs for
+    // host requests (one download).
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t at = off;
         off = align_up(off + bytes);
         return at;
     };
-    const size_t d_x = carve(xs * 4), d_y = carve(ys * 4), d_p0 = carve(p0s * 32);
-    const size_t d_fits = carve(rows * sizeof(bf_fit_desc)), d_grids = carve(nreq * sizeof(bf_grid_desc));
-    const size_t d_cfg = carve(nreq * sizeof(bf_config)), d_info = carve(nreq * sizeof(GridInfo));
-    const size_t d_order = carve(rows * 4), d_cur = carve(kRaggedCounters * 4), d_guess = carve(rows * 32);
-    const size_t d_params = carve(rows * 32), d_status = carve(rows * 4), d_iter = carve(rows * 4),
-                 d_chi2 = carve(rows * 4);
-    const size_t d_cov = unc ? carve(rows * 256) : 0, d_se = unc ? carve(rows * 32) : 0,
-                 d_us = unc ? carve(rows * 4) : 0;
+    const size_t o_x = carve(xs * 4), o_y = carve(ys * 4), o_p0 = carve(p0s * 32);
+    const size_t o_grids = carve(nreq * sizeof(bf_grid_desc));
+    const size_t o_cfg = carve(nreq * sizeof(bf_config)), o_req = carve(nreq * sizeof(ServeReq));
+    const size_t upload = off;
+    const size_t d_fits = carve(rows * sizeof(bf_fit_desc)), d_order = carve(rows * 4);
+    const size_t d_info = carve(nreq * sizeof(GridInfo)), d_cur = carve(kRaggedCounters * 4);
+    const size_t d_guess = carve(rows * 32);
+    const size_t d_out = off;
+    const size_t r_params = carve(rows * 32) - d_out, r_status = carve(rows * 4) - d_out,
+                 r_iter = carve(rows * 4) - d_out, r_chi2 = carve(rows * 4) - d_out;
+    const size_t r_cov = unc ? carve(rows * 256) - d_out : 0,
+                 r_se = unc ? carve(rows * 32) - d_out : 0, r_us = unc ? carve(rows * 4) - d_out : 0;
+    const size_t out_bytes = off - d_out;
     const size_t device_bytes = off;
-    // Host (pinned) layout: staged inputs and tables, then outputs.
-    off = 0;
-    const size_t h_x = carve(xs * 4), h_y = carve(ys * 4), h_p0 = carve(p0s * 32);
-    const size_t h_fits = carve(rows * sizeof(bf_fit_desc)), h_grids = carve(nreq * sizeof(bf_grid_desc));
-    const size_t h_cfg = carve(nreq * sizeof(bf_config)), h_order = carve(rows * 4);
-    slot.out_params = carve(rows * 32);
-    slot.out_status = carve(rows * 4);
-    slot.out_iter = carve(rows * 4);
-    slot.out_chi2 = carve(rows * 4);
-    slot.out_cov = unc ? carve(rows * 256) : 0;
-    slot.out_se = unc ? carve(rows * 32) : 0;
-    slot.out_us = unc ? carve(rows * 4) : 0;
-    const size_t host_bytes = off;
+    const size_t h_out = upload;  // the pinned arena: the prefix, then the outputs
+    const size_t host_bytes = upload + (any_host ? out_bytes : 0);
+    slot.out_params = h_out + r_params;
+    slot.out_status = h_out + r_status;
+    slot.out_iter = h_out + r_iter;
+    slot.out_chi2 = h_out + r_chi2;
+    slot.out_cov = unc ? h_out + r_cov : 0;
+    slot.out_se = unc ? h_out + r_se : 0;
+    slot.out_us = unc ? h_out + r_us : 0;
     auto failed = [&](hipError_t e, const char *what) {
         slot.rc = BF_RUNTIME_ERROR;
         slot.message = std::string(what) + " failed: " + hipGetErrorString(e);
@@ -230,120 +323,129 @@ void Server::launch(Slot &slot) {
     // Device requests are read in place: x and y offsets of every request
     // are taken from the lowest address among the staged region and the
     // device arrays (all float arrays, so every difference is whole floats).
-    uintptr_t xbase = any_host ? uintptr_t(D + d_x) : UINTPTR_MAX;
-    uintptr_t ybase = any_host ? uintptr_t(D + d_y) : UINTPTR_MAX;
+    uintptr_t xbase = any_host ? uintptr_t(D + o_x) : UINTPTR_MAX;
+    uintptr_t ybase = any_host ? uintptr_t(D + o_y) : UINTPTR_MAX;
     for (Request *r : batch)
         if (r->device) {
             xbase = std::min(xbase, uintptr_t(r->x));
             ybase = std::min(ybase, uintptr_t(r->y));
         }
-    auto *fits = reinterpret_cast<bf_fit_desc *>(H + h_fits);
-    auto *grids = reinterpret_cast<bf_grid_desc *>(H + h_grids);
-    auto *cfgs = reinterpret_cast<bf_config *>(H + h_cfg);
-    uint32_t i = 0;
+    // Requests by (model, mapping class), each bucket longest first (stable
+    // in arrival order): the runs of the batch, one per bucket.
+    std::vector<std::vector<uint32_t>> bucket(4 * kRaggedClassCount);
+    for (uint32_t q = 0; q < nreq; ++q)
+        bucket[batch[q]->model * kRaggedClassCount + ragged_class_of(batch[q]->model, batch[q]->m)]
+            .push_back(q);
+    auto *grids = reinterpret_cast<bf_grid_desc *>(H + o_grids);
+    auto *cfgs = reinterpret_cast<bf_config *>(H + o_cfg);
+    auto *reqs = reinterpret_cast<ServeReq *>(H + o_req);
+    RaggedLayout layout;
+    uint32_t pos = 0;
+    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
+        auto &v = bucket[b];
+        if (v.empty())
+            continue;
+        std::stable_sort(v.begin(), v.end(),
+                         [&](uint32_t a, uint32_t c) { return batch[a]->m > batch[c]->m; });
+        const uint32_t model = uint32_t(b / kRaggedClassCount);
+        RaggedRun run{model, b % kRaggedClassCount, pos, 0, 0, 0, 0.0, {}};
+        for (size_t a = 0; a < v.size();) {
+            const uint32_t top = batch[v[a]]->m;
+            const uint32_t below = guess_slice_floor(model, top);
+            const uint32_t slice_at = pos - run.offset;
+            uint32_t count = 0, guessed = 0;
+            for (; a < v.size() && batch[v[a]]->m > below; ++a) {
+                const Request &r = *batch[v[a]];
+                reqs[v[a]].order_pos = pos;
+                pos += r.n;
+                count += r.n;
+                guessed += r.p0 ? 0 : r.n;
+                run.cost += double(r.n) * r.m * kRaggedWeight[model];
+            }
+            if (guessed)
+                run.guess_slices.push_back({slice_at, count, top});
+            run.guesses += guessed;
+            run.count += count;
+            run.max_points = std::max(run.max_points, top);
+        }
+        layout.runs.push_back(std::move(run));
+    }
+    layout.cost_order.resize(layout.runs.size());
+    std::iota(layout.cost_order.begin(), layout.cost_order.end(), size_t(0));
+    std::stable_sort(layout.cost_order.begin(), layout.cost_order.end(), [&](size_t a, size_t c) {
+        return layout.runs[a].cost > layout.runs[c].cost;
+    });
+    layout.out_rows = rows;
     for (uint32_t q = 0; q < nreq; ++q) {
         const Request &r = *batch[q];
-        const uintptr_t xa = r.device ? uintptr_t(r.x) : uintptr_t(D + d_x + r.x_off * 4);
-        const uintptr_t ya = r.device ? uintptr_t(r.y) : uintptr_t(D + d_y + r.y_off * 4);
-        const uint64_t x_off = (xa - xbase) / 4, y_off = (ya - ybase) / 4;
-        grids[q] = bf_grid_desc{x_off, r.m, 0};
+        const uintptr_t xa = r.device ? uintptr_t(r.x) : uintptr_t(D + o_x + r.x_off * 4);
+        const uintptr_t ya = r.device ? uintptr_t(r.y) : uintptr_t(D + o_y + r.y_off * 4);
+        grids[q] = bf_grid_desc{(xa - xbase) / 4, r.m, 0};
         cfgs[q] = r.cfg;
-        for (uint32_t k = 0; k < r.n; ++k, ++i)
-            fits[i] = bf_fit_desc{y_off + uint64_t(k) * r.m, r.m, q, r.model, q,
-                                  r.p0 ? uint32_t(r.p0_row + k) : 0xffffffffu,
-                                  uint32_t(r.out_row + k)};
-    }
-    RaggedLayout layout;
-    const std::string why = ragged_layout(uint32_t(rows), fits, nreq, grids, nreq, layout, false);
-    if (!why.empty()) {
-        slot.rc = BF_INVALID_ARGUMENT;
-        slot.message = why;
-        return;
-    }
-    std::copy(layout.order.begin(), layout.order.end(), reinterpret_cast<uint32_t *>(H + h_order));
-    // Inputs: host requests are staged through pinned memory and sent in one
-    // copy (x, y and p0 rows); device requests' p0 rows are then copied into
-    // the arena (after the bulk copy, in stream order).
-    for (Request *r : batch) {
-        if (r->device)
+        ServeReq &t = reqs[q];
+        t.y_off = (ya - ybase) / 4;
+        t.m = r.m;
+        t.model = r.model;
+        t.first_row = uint32_t(r.out_row);
+        t.n = r.n;
+        t.p0_row = r.p0 ? uint32_t(r.p0_row) : 0xffffffffu;
+        t.P = uint32_t(kParams[r.model]);
+        t.pad = 0;
+        t.p0_dev = r.device ? r.p0 : nullptr;
+        t.params = r.device ? r.params : nullptr;
+        t.chi2 = r.device ? r.chi2 : nullptr;
+        t.status = r.device ? r.status : nullptr;
+        t.iterations = r.device ? r.iterations : nullptr;
+        if (r.device)
             continue;
-        std::memcpy(H + h_x + r->x_off * 4, r->x, size_t(r->m) * 4);
-        std::memcpy(H + h_y + r->y_off * 4, r->y, size_t(r->n) * r->m * 4);
-        if (r->p0) {
-            const int P = kParams[r->model];
-            float *dst = reinterpret_cast<float *>(H + h_p0) + r->p0_row * 8;
-            for (uint32_t k = 0; k < r->n; ++k)
-                std::memcpy(dst + size_t(k) * 8, r->p0 + size_t(k) * P, size_t(P) * 4);
+        std::memcpy(H + o_x + r.x_off * 4, r.x, size_t(r.m) * 4);
+        std::memcpy(H + o_y + r.y_off * 4, r.y, size_t(r.n) * r.m * 4);
+        if (r.p0) {
+            const int P = kParams[r.model];
+            float *dst = reinterpret_cast<float *>(H + o_p0) + r.p0_row * 8;
+            for (uint32_t k = 0; k < r.n; ++k)
+                std::memcpy(dst + size_t(k) * 8, r.p0 + size_t(k) * P, size_t(P) * 4);
         }
     }
-    if ((e = hipMemcpyAsync(D + d_x, H + h_x, h_fits, hipMemcpyHostToDevice, s)) != hipSuccess)
+    if ((e = hipMemcpyAsync(D, H, upload, hipMemcpyHostToDevice, s)) != hipSuccess)
         return failed(e, "staging copy");
-    for (Request *r : batch) {
-        if (!r->device)
-            continue;
-        if (r->p0) {
-            const int P = kParams[r->model];
-            if ((e = hipMemcpy2DAsync(D + d_p0 + r->p0_row * 32, 32, r->p0, size_t(P) * 4,
-                                      size_t(P) * 4, r->n, hipMemcpyDeviceToDevice, s)) !=
-                hipSuccess)
-                return failed(e, "device p0 copy");
-        }
-    }
-    if ((e = hipMemcpyAsync(D + d_fits, H + h_fits, rows * sizeof(bf_fit_desc),
-                            hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(D + d_grids, H + h_grids, nreq * sizeof(bf_grid_desc),
-                            hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(D + d_cfg, H + h_cfg, nreq * sizeof(bf_config), hipMemcpyHostToDevice,
-                            s)) != hipSuccess ||
-        (e = hipMemcpyAsync(D + d_order, H + h_order, rows * 4, hipMemcpyHostToDevice, s)) !=
-            hipSuccess)
-        return failed(e, "table copy");
+    const auto *dreq = reinterpret_cast<const ServeReq *>(D + o_req);
+    const uint32_t blocks = uint32_t((rows + 255) / 256);
+    hipLaunchKernelGGL(serve_expand_kernel, dim3(blocks), dim3(256), 0, s, dreq, nreq,
+                       uint32_t(rows), reinterpret_cast<bf_fit_desc *>(D + d_fits),
+                       reinterpret_cast<uint32_t *>(D + d_order),
+                       reinterpret_cast<float *>(D + o_p0));
+    if ((e = hipGetLastError()) != hipSuccess)
+        return failed(e, "batch expansion");
     const RaggedDevice dev{reinterpret_cast<bf_fit_desc *>(D + d_fits),
-                           reinterpret_cast<bf_grid_desc *>(D + d_grids),
-                           reinterpret_cast<bf_config *>(D + d_cfg),
+                           reinterpret_cast<bf_grid_desc *>(D + o_grids),
+                           reinterpret_cast<bf_config *>(D + o_cfg),
                            reinterpret_cast<GridInfo *>(D + d_info),
                            reinterpret_cast<uint32_t *>(D + d_order),
                            reinterpret_cast<uint32_t *>(D + d_cur),
                            reinterpret_cast<float *>(D + d_guess), nreq};
-    auto F = [&](size_t at) { return reinterpret_cast<float *>(D + at); };
+    char *O = D + d_out;
+    auto F = [&](size_t at) { return reinterpret_cast<float *>(O + at); };
     const RaggedIO io{reinterpret_cast<const float *>(xbase),
-                      reinterpret_cast<const float *>(ybase), F(d_p0), F(d_params),
-                      reinterpret_cast<int32_t *>(D + d_status),
-                      reinterpret_cast<uint32_t *>(D + d_iter), F(d_chi2),
-                      unc ? F(d_cov) : nullptr, unc ? F(d_se) : nullptr,
-                      unc ? reinterpret_cast<int32_t *>(D + d_us) : nullptr};
+                      reinterpret_cast<const float *>(ybase),
+                      reinterpret_cast<const float *>(D + o_p0), F(r_params),
+                      reinterpret_cast<int32_t *>(O + r_status),
+                      reinterpret_cast<uint32_t *>(O + r_iter), F(r_chi2),
+                      unc ? F(r_cov) : nullptr, unc ? F(r_se) : nullptr,
+                      unc ? reinterpret_cast<int32_t *>(O + r_us) : nullptr};
     const RaggedFork fork{Slot::kBranches, slot.side, slot.fork, slot.join};
     if ((e = ragged_enqueue(layout, dev, mixed, io, s, &fork)) != hipSuccess)
         return failed(e, "batch enqueue");
-    // Outputs: device requests receive device-to-device copies in stream
-    // order; host requests are copied back to pinned memory and scattered on
-    // completion.
-    for (Request *r : batch) {
-        if (!r->device)
-            continue;
-        const int P = kParams[r->model];
-        const uint64_t r0 = r->out_row;
-        if ((e = hipMemcpy2DAsync(r->params, size_t(P) * 4, D + d_params + r0 * 32, 32,
-                                  size_t(P) * 4, r->n, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(r->status, D + d_status + r0 * 4, size_t(r->n) * 4,
-                                hipMemcpyDeviceToDevice, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(r->iterations, D + d_iter + r0 * 4, size_t(r->n) * 4,
-                                hipMemcpyDeviceToDevice, s)) != hipSuccess ||
-            (e = hipMemcpyAsync(r->chi2, D + d_chi2 + r0 * 4, size_t(r->n) * 4,
-                                hipMemcpyDeviceToDevice, s)) != hipSuccess)
-            return failed(e, "device output copy");
+    if (any_device) {
+        hipLaunchKernelGGL(serve_scatter_kernel, dim3(blocks), dim3(256), 0, s, dreq, nreq,
+                           uint32_t(rows), F(r_params),
+                           reinterpret_cast<const int32_t *>(O + r_status),
+                           reinterpret_cast<const uint32_t *>(O + r_iter), F(r_chi2));
+        if ((e = hipGetLastError()) != hipSuccess)
+            return failed(e, "device output scatter");
     }
-    auto d2h = [&](size_t dst, size_t src, size_t bytes) {
-        return hipMemcpyAsync(H + dst, D + src, bytes, hipMemcpyDeviceToHost, s);
-    };
-    if ((e = d2h(slot.out_params, d_params, rows * 32)) != hipSuccess ||
-        (e = d2h(slot.out_status, d_status, rows * 4)) != hipSuccess ||
-        (e = d2h(slot.out_iter, d_iter, rows * 4)) != hipSuccess ||
-        (e = d2h(slot.out_chi2, d_chi2, rows * 4)) != hipSuccess)
-        return failed(e, "output copy");
-    if (unc && ((e = d2h(slot.out_cov, d_cov, rows * 256)) != hipSuccess ||
-                (e = d2h(slot.out_se, d_se, rows * 32)) != hipSuccess ||
-                (e = d2h(slot.out_us, d_us, rows * 4)) != hipSuccess))
+    if (any_host &&
+        (e = hipMemcpyAsync(H + h_out, O, out_bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
         return failed(e, "output copy");
     if ((e = hipEventRecord(slot.done, s)) != hipSuccess)
         return failed(e, "hipEventRecord");
