@@ -8,6 +8,8 @@
 * Errors: argument errors raise at submission; a batch's data failures stay in
   its status rows.
 """
+import os
+import subprocess
 import threading
 
 import numpy as np
@@ -141,3 +143,19 @@ def test_submission_errors(hip):
     with Server(hbm_budget=4096) as small:
         with pytest.raises(ValueError, match="budget"):
             small.submit(x, np.ones((100, 64), np.float32), "gaussian")
+
+
+def test_server_host_code_under_address_sanitizer():
+    """The server's host code built with AddressSanitizer and UBSan
+    (tests/cpp/Makefile, target asan, part of __graft_entry__.build()) and
+    driven by tests/cpp/serve_stress.cpp: 400 host and device requests from
+    four threads while a fifth polls the counters, batches capped at 512 fits
+    so admission and completion interleave; no sanitizer report, and every
+    result equals bf_hip_fit on the same request."""
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "_build", "serve_stress_asan")
+    assert os.path.exists(exe), "build it with `make -C tests/cpp asan`"
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    out = subprocess.run([exe, "400"], capture_output=True, text=True, env=env, timeout=240)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-4000:]
+    assert "mismatches 0" in out.stdout and "Sanitizer" not in out.stderr, out.stderr[-4000:]
