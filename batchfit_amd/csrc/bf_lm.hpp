@@ -157,10 +157,10 @@ __device__ __forceinline__ bool chol_solve(S *a, S *b, bool scaled) {
 }
 
 // Per-lane partial accumulation. Plain: one FMA per term and point.
-// Compensated: FMA into a block partial over 32 points of the lane (8 for the
-// two-peak model), then the block partial is added into a Kahan pair (sum,
-// carry) (numerics.hpp:12-18), so the compensation costs 4 VALU per term per
-// block instead of per point. A lane's partial is (sum − carry).
+// Compensated: FMA into a block partial over 32 points of the lane, then the
+// block partial is added into a Kahan pair (sum, carry) (numerics.hpp:12-18),
+// so the compensation costs 4 VALU per term per block instead of per point.
+// A lane's partial is (sum − carry).
 //
 // Flush kinds: FlushClear adds the block into the pair and clears the block
 // partial; FlushKeep adds it and leaves the partial for the next block's
@@ -301,7 +301,7 @@ template <int G, int R, int CHUNK = 16> struct Points {
     // f(x, y) for every point this lane owns, then flush() after every block
     // of kBlock points. Slots below m/G are full for every lane of the group
     // (a scalar branch); only slot m/G is partial.
-    static constexpr int kBlock = CHUNK < 16 ? CHUNK : 32;
+    static constexpr int kBlock = 32;
     static constexpr int kChunk = CHUNK;
     template <typename F, typename Flush>
     __device__ __forceinline__ void each(F &&f, Flush &&flush) const {
