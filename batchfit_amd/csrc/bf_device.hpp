@@ -500,6 +500,13 @@ struct GridInfo {
     float guess_spacing;    // min (x[j] - x[j-1])
     float left_x, right_x;  // midpoints of the two edge windows
     int32_t count;          // edge window length max(1, M/10)
+    float max_abs_x;        // largest |x[j]| (+inf if some x is not finite)
 };
+
+// Grids of |x| up to this bound take the fixed-baseline form of the two-peak
+// LM kernels: there the baseline columns' sums (Σ J_k, Σ J_k·x, Σ r, Σ r·x,
+// Σ 1, Σ x, Σ x²) are finite whenever the free columns' sums and χ² are
+// (Cauchy–Schwarz), so leaving them out changes no finiteness test.
+constexpr float kFixedBaselineMaxX = 1.0e6f;
 
 }  // namespace bfhip

@@ -35,12 +35,28 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
                 // second launch, one group per fit.
                 const uint32_t waves = (args.n + pool_refill(G) - 1) / pool_refill(G);
                 const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
+                FitArgs full = args;
+                if constexpr (MODEL == TWO_LORENTZIAN) {
+                    // Fixed baseline: the form that sums only the free
+                    // parameters' rows, then the full form for wide grids.
+                    if (args.lower[6] == args.upper[6] && args.lower[7] == args.upper[7]) {
+                        if (args.flat_check)
+                            hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, true, true>),
+                                               dim3(pool_blocks), dim3(256), 0, stream, args);
+                        else
+                            hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, false, true>),
+                                               dim3(pool_blocks), dim3(256), 0, stream, args);
+                        if (hipError_t e = hipGetLastError(); e != hipSuccess)
+                            return e;
+                        full.wide_grid_only = 1;
+                    }
+                }
                 if (args.flat_check)
                     hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, true>),
-                                       dim3(pool_blocks), dim3(256), 0, stream, args);
+                                       dim3(pool_blocks), dim3(256), 0, stream, full);
                 else
                     hipLaunchKernelGGL((lm_pool_kernel<MODEL, G, COMP, MIXED, false>),
-                                       dim3(pool_blocks), dim3(256), 0, stream, args);
+                                       dim3(pool_blocks), dim3(256), 0, stream, full);
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess || !args.cov)
                     return e;

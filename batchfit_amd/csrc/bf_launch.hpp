@@ -47,14 +47,16 @@ template <int MODEL> __global__ void __launch_bounds__(256) model_eval_kernel(Ev
 // initial_guess.hpp:131-153) for one 256-thread block. Minima are exact, so the
 // parallel order gives the reference's sequential values.
 __device__ inline void grid_block(const float *x, uint32_t m, GridInfo *out) {
-    __shared__ float s_abs[256], s_gap[256];
+    __shared__ float s_abs[256], s_gap[256], s_max[256];
     __shared__ int s_flags[256];
     const int t = threadIdx.x;
     const float x0 = x[0];
     float amin = __uint_as_float(0x7f800000u), gmin = 3.40282346638528859812e+38f;
+    float xmax = 0.0f;  // largest finite |x| (a non-finite x sets flag 1)
     int fl = 0;  // 1 non-finite, 2 not rising, 4 not falling, 8 varied, 16 not strictly rising
     for (uint32_t j = t; j < m; j += 256) {
         const float xj = x[j];
+        xmax = fmaxf(xmax, fabsf(xj));
         fl |= finite_f(xj) ? 0 : 1;
         fl |= (xj != x0) ? 8 : 0;
         if (j > 0) {
@@ -70,12 +72,14 @@ __device__ inline void grid_block(const float *x, uint32_t m, GridInfo *out) {
     }
     s_abs[t] = amin;
     s_gap[t] = gmin;
+    s_max[t] = xmax;
     s_flags[t] = fl;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (t < o) {
             s_abs[t] = fminf(s_abs[t], s_abs[t + o]);
             s_gap[t] = fminf(s_gap[t], s_gap[t + o]);
+            s_max[t] = fmaxf(s_max[t], s_max[t + o]);
             s_flags[t] |= s_flags[t + o];
         }
         __syncthreads();
@@ -93,6 +97,7 @@ __device__ inline void grid_block(const float *x, uint32_t m, GridInfo *out) {
         g.count = int(count);
         g.left_x = std_midpoint(x[(count - 1) / 2], x[count / 2]);
         g.right_x = std_midpoint(x[m - count + (count - 1) / 2], x[m - count + count / 2]);
+        g.max_abs_x = (f & 1) ? __uint_as_float(0x7f800000u) : s_max[0];
         *out = g;
     }
 }

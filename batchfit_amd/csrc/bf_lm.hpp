@@ -59,6 +59,8 @@ struct FitArgs {
     int32_t free_shape;  // cpu/solver.hpp:70-82
     int32_t flat_check;  // some fit may have flat observations (its p0 is not a device guess)
     float width_factor;  // cpu/solver.hpp:86-88
+    int32_t wide_grid_only;  // 1: a full two-peak pool kernel that runs only where the
+                             // fixed-baseline form does not (grids beyond kFixedBaselineMaxX)
 };
 
 __device__ __forceinline__ float clampf(float v, float lo, float hi) {
@@ -405,18 +407,30 @@ template <int G, int R, int CHUNK = 16> struct Points {
     }
 };
 
+// Index in the full term layout (packed JᵀJ, Jᵀr, χ² of P parameters) of term
+// t of the layout of the first PA parameters (the packed triangle of the
+// first PA rows is the start of the full one).
+template <int P, int PA> __host__ __device__ constexpr int full_term(int t) {
+    constexpr int TRI = P * (P + 1) / 2, TRIA = PA * (PA + 1) / 2;
+    return t < TRIA ? t : t < TRIA + PA ? TRI + (t - TRIA) : TRI + P;
+}
+
 // One pass at parameters p: χ² (terms[TRI+P]), JᵀJ packed (terms[0..TRI)),
 // Jᵀr (terms[TRI..TRI+P)), reduced over the group. `lin_ok` is the finiteness
-// check of the linearisation (cpu/solver.hpp:182-190).
-template <int MODEL, int G, int R, bool COMP, typename S>
+// check of the linearisation (cpu/solver.hpp:182-190). Only the first PA
+// parameters' rows are summed (PA < P for the fixed-baseline form of the
+// two-peak model); the other rows' terms are written as 0 and take no part in
+// `lin_ok` (the caller guarantees they are never used and always finite).
+template <int MODEL, int G, int R, bool COMP, typename S, int PA = Params<MODEL>::P>
 __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL)> &pts, const float *p, S *terms,
                                            bool &lin_ok, float *yrange = nullptr) {
-    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI, T = Params<MODEL>::TERMS;
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    constexpr int TRIA = PA * (PA + 1) / 2, TA = TRIA + PA + 1;
     ModelConsts<MODEL> mc;
     mc.set(p);
-    Acc<COMP> acc[T];
+    Acc<COMP> acc[TA];
 #pragma unroll
-    for (int t = 0; t < T; ++t)
+    for (int t = 0; t < TA; ++t)
         acc[t].zero();
     // With yrange: also the group's smallest and largest observation.
     float ylo = __uint_as_float(0x7f800000u), yhi = -__uint_as_float(0x7f800000u);
@@ -429,16 +443,16 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         const float f = mc.eval(xj, jac);
         const float r = yj - f;
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < PA; ++k) {
 #pragma unroll
             for (int l = 0; l <= k; ++l)
                 acc[tri(k, l)].add_prod(jac[k], jac[l], first);
-            acc[TRI + k].add_prod(jac[k], r, first);
+            acc[TRIA + k].add_prod(jac[k], r, first);
         }
-        acc[TRI + P].add_prod(r, r, first);
+        acc[TRIA + PA].add_prod(r, r, first);
     }, [&](auto kind) {
 #pragma unroll
-        for (int t = 0; t < T; ++t)
+        for (int t = 0; t < TA; ++t)
             acc[t].flush(kind);
     });
     if (yrange) {
@@ -450,26 +464,36 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
         yrange[1] = yhi;
     }
     lin_ok = true;
+    if constexpr (PA < P) {
+#pragma unroll
+        for (int t = TRIA; t < TRI; ++t)
+            terms[t] = S(0);
+#pragma unroll
+        for (int k = PA; k < P; ++k)
+            terms[TRI + k] = S(0);
+    }
     if constexpr (sizeof(S) == 8) {
         // Each lane's (sum, carry) pair as one double, then one halving
         // all-reduce over the group for all terms together.
-        double v[T];
+        double v[TA];
 #pragma unroll
-        for (int t = 0; t < T; ++t)
+        for (int t = 0; t < TA; ++t)
             v[t] = double(acc[t].hi()) - double(acc[t].lo());
-        group_allreduce_d<G, T>(v, pts.gl);
+        group_allreduce_d<G, TA>(v, pts.gl);
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            terms[t] = v[t] * double(term_scale<MODEL>(t));
-            if (t < TRI + P)
-                lin_ok = lin_ok && finite_s(terms[t]);
+        for (int t = 0; t < TA; ++t) {
+            const int u = full_term<P, PA>(t);
+            terms[u] = v[t] * double(term_scale<MODEL>(u));
+            if (t < TRIA + PA)
+                lin_ok = lin_ok && finite_s(terms[u]);
         }
     } else {
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            terms[t] = reduce_term<G, S, COMP>(acc[t]) * term_scale<MODEL>(t);
-            if (t < TRI + P)
-                lin_ok = lin_ok && finite_s(terms[t]);
+        for (int t = 0; t < TA; ++t) {
+            const int u = full_term<P, PA>(t);
+            terms[u] = reduce_term<G, S, COMP>(acc[t]) * term_scale<MODEL>(u);
+            if (t < TRIA + PA)
+                lin_ok = lin_ok && finite_s(terms[u]);
         }
     }
 }
@@ -874,9 +898,17 @@ constexpr uint32_t kNoFit = 0xffffffffu;
 // successful device guess cannot have flat observations (the guess needs a
 // smoothed peak above three noise scales), so launches whose every initial
 // point is a guess leave the range out of their passes.
-template <int MODEL, int G, bool COMP, bool FLAT, typename S, typename Source>
+//
+// FIXB (two-peak model, both baseline parameters fixed by equal bounds — the
+// caller guarantees it — on a grid within kFixedBaselineMaxX): the baseline
+// parameters are pinned at compile time, so their rows of JᵀJ and their
+// gradient, which a pinned parameter never reads (identity rows in the damped
+// matrix, zero steps), are not summed, and the predicted gain leaves out
+// their terms, which are exactly zero (a fixed parameter's step is 0).
+template <int MODEL, int G, bool COMP, bool FLAT, typename S, bool FIXB = false, typename Source>
 __device__ __forceinline__ void solve_pool(int gl, Source &src) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    constexpr int PA = (FIXB && MODEL == TWO_LORENTZIAN) ? 6 : P;  // parameters summed
     constexpr float eps = 1.1920928955078125e-7f;
     // A group that never gets a fit still takes part in every pass, over zero
     // points: its lane index must be set, since the pass loops start there.
@@ -939,7 +971,8 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
                 S gnorm = S(0);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    pinned[k] = a.lower[k] == a.upper[k] || (p[k] == a.lower[k] && g[k] < S(0)) ||
+                    pinned[k] = k >= PA || a.lower[k] == a.upper[k] ||
+                                (p[k] == a.lower[k] && g[k] < S(0)) ||
                                 (p[k] == a.upper[k] && g[k] > S(0));
                     if (!pinned[k])
                         gnorm = smax(gnorm, sabs(g[k]));
@@ -978,10 +1011,10 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
                         } else {
                             predicted = S(0);
 #pragma unroll
-                            for (int k = 0; k < P; ++k) {
+                            for (int k = 0; k < PA; ++k) {
                                 predicted += S(2) * g[k] * step[k];
 #pragma unroll
-                                for (int l = 0; l < P; ++l)
+                                for (int l = 0; l < PA; ++l)
                                     predicted -= step[k] * A[k >= l ? tri(k, l) : tri(l, k)] * step[l];
                             }
                             pass = finite_trial;
@@ -1019,7 +1052,7 @@ __device__ __forceinline__ void solve_pool(int gl, Source &src) {
         S terms[Params<MODEL>::TERMS];
         bool pass_ok;
         float yrange[2];
-        fused_pass<MODEL, G, 0, COMP, S>(due, trial, terms, pass_ok, FLAT ? yrange : nullptr);
+        fused_pass<MODEL, G, 0, COMP, S, PA>(due, trial, terms, pass_ok, FLAT ? yrange : nullptr);
         // 4. After the pass (no early exits: every path reaches the next trip's
         //    test the same way).
         if (fit != kNoFit) {
@@ -1171,13 +1204,19 @@ constexpr int lm_min_waves(int model, int g, int r) {
 }
 
 // Streamed groups with retiring fits (no uncertainties): persistent waves over
-// the batch counter a.pool.
-template <int MODEL, int G, bool COMP, bool MIXED, bool FLAT>
+// the batch counter a.pool. A two-peak batch with a fixed baseline launches
+// the FIXB form and then the full form with wide_grid_only set on the same
+// counter: each reads the grid facts first, and exactly one of them runs
+// (the FIXB form on grids within kFixedBaselineMaxX).
+template <int MODEL, int G, bool COMP, bool MIXED, bool FLAT, bool FIXB = false>
 __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, 0)) lm_pool_kernel(FitArgs a) {
     using S = typename std::conditional<MIXED, double, float>::type;
+    const bool narrow = a.grid->max_abs_x <= kFixedBaselineMaxX;
+    if (FIXB ? !narrow : (a.wide_grid_only && narrow))
+        return;  // the whole launch: the other form serves this grid
     const int lane = int(threadIdx.x & 63);
     BatchPool<G> src(a, lane, lane % G);
-    solve_pool<MODEL, G, COMP, FLAT, S>(lane % G, src);
+    solve_pool<MODEL, G, COMP, FLAT, S, FIXB>(lane % G, src);
 }
 
 // Uncertainties after a retiring launch: one group per fit, from the fit's

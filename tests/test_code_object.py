@@ -89,6 +89,16 @@ def test_guess_kernels_use_no_private_memory(resources):
 
 
 def test_metric_lm_kernel_has_no_spills(resources):
-    name = "_ZN5bfhip14lm_pool_kernelILi2ELi8ELb1ELb1ELb0EEEvNS_7FitArgsE"
+    name = "_ZN5bfhip14lm_pool_kernelILi2ELi8ELb1ELb1ELb0ELb0EEEvNS_7FitArgsE"
     assert name in resources
     assert resources[name].get("private", 0) == 0 and resources[name].get("spill", 0) == 0
+
+
+def test_fixed_baseline_two_peak_kernels_do_not_spill(resources):
+    """The two-peak pool kernels that sum only the six free parameters' rows
+    (baseline fixed) keep their state in registers: no private segment."""
+    fixb = {k: v for k, v in resources.items()
+            if k.startswith("_ZN5bfhip14lm_pool_kernelILi3E") and k.endswith("Lb1EEEvNS_7FitArgsE")}
+    assert len(fixb) >= 5, sorted(fixb)
+    for name, r in fixb.items():
+        assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)

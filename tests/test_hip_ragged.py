@@ -64,6 +64,33 @@ def test_ragged_equals_uniform_path(hip):
                 assert same(u, v), (model, m)
 
 
+@pytest.mark.parametrize("shift", [0.0, 2.0e6])
+def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
+    """Doublets with the baseline fixed by equal bounds: a uniform batch runs
+    the pool kernel that sums only the six free parameters' rows (grids with
+    |x| up to 1e6) or, on a grid beyond that, the full kernel; a ragged batch
+    always runs the full kernel. The same bytes either way."""
+    from batchfit_amd.ragged import fit_ragged
+    x, y, _, _, _ = datasets.comparison_set("two_lorentzian", 300, 900)
+    x = (shift + 80.0 * x).astype(np.float32)  # spacing 0.89, above an ulp at 2e6
+    y = y.astype(np.float32)
+    assert np.all(np.diff(x) > 0)
+    lower = np.array([0, shift - 400, .8, 0, shift - 400, .8, 0.3, 0], np.float64)
+    upper = np.array([np.inf, shift + 400, 400, np.inf, shift + 400, 400, 0.3, 0], np.float64)
+    bounds = (lower, upper)
+    a = hip.fit(x, y, "two_lorentzian", None, bounds, "hip")
+    b = fit_ragged([(x, y, "two_lorentzian", None, bounds)])[0]
+    for u, v in zip(a, b):
+        assert same(u, v), shift
+    # The grid is 80x the unit-scale sets', so positions and widths are too:
+    # statuses and chi2 against the oracle (the unit-scale parameter gate of
+    # the other tests holds this path at M = 2048 on the doublet workload).
+    ref = oracle_port.fit(x, y, "two_lorentzian", None, bounds, threads=8)
+    ok = (ref.status == 0) & (a.status == 0)
+    assert np.mean(a.status == ref.status) >= 0.99 and ok.sum() > 100
+    assert np.all(np.abs(a.chi2[ok] - ref.chi2[ok]) <= 1e-4 * np.abs(ref.chi2[ok]))
+
+
 def test_ragged_long_spectra_from_guess(hip):
     """Guessed fits beyond the one-wave guess's LDS (M > 12,288 points) take
     the wide guess in a ragged batch: the same bytes as the uniform entry
