@@ -37,6 +37,9 @@ struct RaggedArgs {
     float *cov;                 // rows of 64 or null
     float *stderr_;             // rows of 8
     int32_t *ustatus;
+    const uint32_t *narrow;     // 1 when every grid of the batch lies within kFixedBaselineMaxX
+    uint32_t fixed_baseline;    // two-peak class whose every fit fixes both baseline parameters
+    uint32_t wide_only;         // the full form of such a class: runs only when !*narrow
 };
 
 // The value of lane 0 in every lane, held in scalar registers (for data a
@@ -198,14 +201,19 @@ template <int MODEL, int G> struct RaggedPool : PoolQueue<G> {
 };
 
 // A streamed class without uncertainties: persistent waves whose groups
-// retire fits one by one (solve_pool) from the class's counter.
-template <int MODEL, int G, bool MIXED, bool FLAT>
+// retire fits one by one (solve_pool) from the class's counter. A two-peak
+// class with a fixed baseline launches the FIXB form and then the full form
+// with wide_only set, on the same counter: exactly one of them runs (the FIXB
+// form when every grid of the batch lies within kFixedBaselineMaxX).
+template <int MODEL, int G, bool MIXED, bool FLAT, bool FIXB = false>
 __global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, 0))
     lm_ragged_pool_kernel(RaggedArgs r) {
     using S = typename std::conditional<MIXED, double, float>::type;
+    if (FIXB ? !*r.narrow : (r.wide_only && *r.narrow))
+        return;  // the whole launch: the other form serves this class
     const int lane = int(threadIdx.x & 63);
     RaggedPool<MODEL, G> src(r, lane, lane % G);
-    solve_pool<MODEL, G, MIXED, FLAT, S>(lane % G, src);
+    solve_pool<MODEL, G, MIXED, FLAT, S, FIXB>(lane % G, src);
 }
 
 // Uncertainties of a streamed class after its retiring launch: one group per
@@ -275,8 +283,12 @@ constexpr RaggedClass kRaggedClasses[kRaggedClassCount] = {
     {4, 0, 0xffffffffu}};
 // Enough queue counters for every run: one per (model, class) bucket, plus
 // at most 8 more from splitting buckets heavier than an eighth of the batch.
-constexpr int kRaggedCounters = 64;
-static_assert(4 * kRaggedClassCount + 8 <= kRaggedCounters, "one counter per run");
+// Queue counters: one per (model, class) bucket and one per fixed-baseline
+// two-peak class, plus at most 8 more from splitting heavy buckets; the word
+// after them is the batch's narrow-grid flag.
+constexpr int kRaggedCounters = 80;
+static_assert(5 * kRaggedClassCount + 8 <= kRaggedCounters, "one counter per run");
+constexpr int kRaggedCounterWords = kRaggedCounters + 1;
 inline int ragged_class_of(uint32_t model, uint32_t m) {
     const TunedRange t = tuned_mapping(int(model), m);
     // Beyond 32 points per lane the points stream, as in a uniform launch.
@@ -301,12 +313,19 @@ struct GuessSlice {
 struct RaggedRun {
     uint32_t model;
     int cls;
+    bool fixed_baseline;  // two-peak fits that all fix both baseline parameters
     uint32_t offset, count;  // slice of the order array
     uint32_t guesses;        // fits in the slice that start from the device guess
     uint32_t max_points;
     double cost;             // relative work, for ordering and branch balance
     std::vector<GuessSlice> guess_slices;  // octaves that hold a guessed fit
 };
+
+// Whether a fit of `model` under config c runs in the fixed-baseline form:
+// two-peak, both baseline parameters fixed by equal bounds.
+inline bool fixed_baseline(uint32_t model, const bf_config &c) {
+    return model == TWO_LORENTZIAN && c.lower[6] == c.upper[6] && c.lower[7] == c.upper[7];
+}
 
 // Relative work per point of a fit by model (points × parameter-weighted
 // passes: a P=8 fit costs about three times a P=3 one per point), for run
@@ -345,7 +364,7 @@ struct RaggedDevice {
     const bf_config *configs;
     GridInfo *info;
     const uint32_t *order;
-    uint32_t *cursors;  // ≥ runs.size() counters
+    uint32_t *cursors;  // kRaggedCounterWords words: the runs' counters, then the narrow-grid flag
     float *guess;       // out_rows × 8 when any fit starts from the guess
     uint32_t grid_count;
 };
@@ -355,8 +374,8 @@ struct RaggedDevice {
 // (for the branches of a large batch's graph; the server's batches stay
 // whole, which kept their tail latency lower).
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
-                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out,
-                          bool split);
+                          const bf_grid_desc *grid_table, uint32_t configs,
+                          const bf_config *config_table, RaggedLayout &out, bool split);
 
 // Device inputs and outputs of one ragged batch (rows of 8 for params and
 // stderr, 64 for cov; cov/stderr/ustatus null without uncertainties).

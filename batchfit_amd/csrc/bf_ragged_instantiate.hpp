@@ -77,12 +77,28 @@ hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream
         // at most 4096 waves), then uncertainties one group per fit.
         const uint32_t waves = (args.count + pool_refill(G) - 1) / pool_refill(G);
         const uint32_t pool_blocks = std::min<uint32_t>((waves + 3) / 4, 1024u);
+        RaggedArgs full = args;
+        if constexpr (MODEL == TWO_LORENTZIAN) {
+            // Fixed baseline: the form that sums only the free parameters'
+            // rows, then the full form for batches with a wide grid.
+            if (args.fixed_baseline) {
+                if (args.flat_check)
+                    hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, true, true>),
+                                       dim3(pool_blocks), dim3(256), 0, stream, args);
+                else
+                    hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, false, true>),
+                                       dim3(pool_blocks), dim3(256), 0, stream, args);
+                if (hipError_t e = hipGetLastError(); e != hipSuccess)
+                    return e;
+                full.wide_only = 1;
+            }
+        }
         if (args.flat_check)
             hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, true>), dim3(pool_blocks),
-                               dim3(256), 0, stream, args);
+                               dim3(256), 0, stream, full);
         else
             hipLaunchKernelGGL((lm_ragged_pool_kernel<MODEL, G, MIXED, false>), dim3(pool_blocks),
-                               dim3(256), 0, stream, args);
+                               dim3(256), 0, stream, full);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || !args.cov)
             return e;

@@ -15,15 +15,22 @@ namespace bfhip {
 
 namespace {
 
+// Queue counters of the runs cleared, and the batch's narrow-grid flag (the
+// word after the counters) set; grid_table_kernel clears the flag for a grid
+// beyond kFixedBaselineMaxX.
 __global__ void __launch_bounds__(64) reset_kernel(uint32_t *cursors, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += 64)
         cursors[i] = 0;
+    if (threadIdx.x == 0)
+        cursors[kRaggedCounters] = 1;
 }
 
 __global__ void __launch_bounds__(256) grid_table_kernel(const bf_grid_desc *grids, const float *x,
-                                                         GridInfo *info) {
+                                                         GridInfo *info, uint32_t *narrow) {
     const bf_grid_desc g = grids[blockIdx.x];
     grid_block(x + g.x_offset, g.points, info + blockIdx.x);
+    if (threadIdx.x == 0 && !(info[blockIdx.x].max_abs_x <= kFixedBaselineMaxX))
+        *narrow = 0;  // every writer stores 0
 }
 
 hipError_t ragged_lm(uint32_t model, int cls, bool mixed, const RaggedArgs &a, uint32_t blocks,
@@ -85,15 +92,17 @@ void sort_longest_first(std::vector<uint32_t> &v, const bf_fit_desc *t) {
 }  // namespace
 
 std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t grids,
-                          const bf_grid_desc *grid_table, uint32_t configs, RaggedLayout &out,
-                          bool split) {
+                          const bf_grid_desc *grid_table, uint32_t configs,
+                          const bf_config *config_table, RaggedLayout &out, bool split) {
     out = RaggedLayout{};
-    if (fits && (!fit_table || !grid_table || !grids || !configs))
+    if (fits && (!fit_table || !grid_table || !grids || !configs || !config_table))
         return "descriptor tables are required";
     for (uint32_t g = 0; g < grids; ++g)
         if (grid_table[g].points == 0)
             return "every grid needs M>0";
-    std::vector<std::vector<uint32_t>> bucket(4 * kRaggedClassCount);
+    // Buckets: (model, class), and for the two-peak model a second set of
+    // classes for fits that fix both baseline parameters.
+    std::vector<std::vector<uint32_t>> bucket(5 * kRaggedClassCount);
     for (uint32_t i = 0; i < fits; ++i) {
         const bf_fit_desc &d = fit_table[i];
         if (d.model > 3 || d.grid >= grids || d.config >= configs ||
@@ -105,7 +114,10 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         out.any_guess = out.any_guess || d.p0_row == 0xffffffffu;
         out.any_p0 = out.any_p0 || d.p0_row != 0xffffffffu;
         out.out_rows = std::max(out.out_rows, uint64_t(d.out_row) + 1);
-        bucket[d.model * kRaggedClassCount + ragged_class_of(d.model, d.points)].push_back(i);
+        const int cls = ragged_class_of(d.model, d.points);
+        bucket[(fixed_baseline(d.model, config_table[d.config]) ? 4u : d.model) *
+                   kRaggedClassCount + cls]
+            .push_back(i);
     }
     // Two fits writing one output row would race on it (and on its guess
     // row): every fit needs a row of its own.
@@ -130,8 +142,8 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
     // than the balance gains.
     const double piece_cost =
         split ? std::max(total / 8.0, 67108864.0) : std::numeric_limits<double>::infinity();
-    auto add_run = [&](uint32_t model, int cls, const uint32_t *v, size_t n) {
-        RaggedRun r{model, cls, uint32_t(out.order.size()), uint32_t(n), 0, 0, 0.0, {}};
+    auto add_run = [&](uint32_t model, int cls, bool fixb, const uint32_t *v, size_t n) {
+        RaggedRun r{model, cls, fixb, uint32_t(out.order.size()), uint32_t(n), 0, 0, 0.0, {}};
         double points = 0.0;
         for (size_t k = 0; k < n; ++k) {
             r.guesses += fit_table[v[k]].p0_row == 0xffffffffu;
@@ -154,24 +166,25 @@ std::string ragged_layout(uint32_t fits, const bf_fit_desc *fit_table, uint32_t 
         out.order.insert(out.order.end(), v, v + n);
         out.runs.push_back(std::move(r));
     };
-    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
+    for (int b = 0; b < 5 * kRaggedClassCount; ++b) {
         auto &v = bucket[b];
         if (v.empty())
             continue;
         sort_longest_first(v, fit_table);
-        const uint32_t model = uint32_t(b / kRaggedClassCount);
+        const bool fixb = b >= 4 * kRaggedClassCount;
+        const uint32_t model = fixb ? uint32_t(BF_TWO_LORENTZIAN) : uint32_t(b / kRaggedClassCount);
         const int cls = b % kRaggedClassCount;
         size_t a = 0;
         double acc = 0.0;
         for (size_t k = 0; k < v.size(); ++k) {
             acc += double(fit_table[v[k]].points) * kRaggedWeight[model];
             if (acc >= piece_cost && k + 1 < v.size()) {
-                add_run(model, cls, v.data() + a, k + 1 - a);
+                add_run(model, cls, fixb, v.data() + a, k + 1 - a);
                 a = k + 1;
                 acc = 0.0;
             }
         }
-        add_run(model, cls, v.data() + a, v.size() - a);
+        add_run(model, cls, fixb, v.data() + a, v.size() - a);
     }
     if (out.runs.size() > size_t(kRaggedCounters))
         return "internal: more runs than queue counters";
@@ -238,6 +251,9 @@ hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice 
     a.cov = io.cov;
     a.stderr_ = io.stderr_;
     a.ustatus = io.ustatus;
+    a.narrow = dev.cursors + kRaggedCounters;
+    a.fixed_baseline = r.fixed_baseline;
+    a.wide_only = 0;
     const uint32_t per_wave = 64 / uint32_t(kRaggedClasses[r.cls].G);
     const uint32_t waves = (r.count + per_wave - 1) / per_wave;
     const uint32_t blocks = std::min<uint32_t>((waves + 3) / 4, 256u * 8u);
@@ -254,7 +270,7 @@ hipError_t ragged_enqueue(const RaggedLayout &layout, const RaggedDevice &dev, b
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     hipLaunchKernelGGL(grid_table_kernel, dim3(dev.grid_count), dim3(256), 0, s, dev.grids, io.x,
-                       dev.info);
+                       dev.info, dev.cursors + kRaggedCounters);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     const int branches = fork ? std::min<int>(fork->streams, int(layout.runs.size())) : 0;
@@ -396,7 +412,7 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
         return fail(err, errlen, BF_INVALID_ARGUMENT, "descriptor tables are required");
     auto p = new bf_ragged_plan_s();
     const std::string why =
-        ragged_layout(fits, fit_table, grids, grid_table, configs, p->layout, true);
+        ragged_layout(fits, fit_table, grids, grid_table, configs, config_table, p->layout, true);
     if (!why.empty()) {
         delete p;
         return fail(err, errlen, BF_INVALID_ARGUMENT, why);
@@ -434,7 +450,7 @@ int bf_hip_ragged_create(bf_ragged_plan *plan, uint32_t fits, const bf_fit_desc 
         hipSuccess)
         return bad(e, "hipMalloc(grid info)");
     if ((e = hipMalloc(reinterpret_cast<void **>(&p->d_cursors),
-                       sizeof(uint32_t) * kRaggedCounters)) != hipSuccess)
+                       sizeof(uint32_t) * kRaggedCounterWords)) != hipSuccess)
         return bad(e, "hipMalloc(queue counters)");
     if (p->any_guess &&
         (e = hipMalloc(reinterpret_cast<void **>(&p->d_guess),

@@ -287,7 +287,7 @@ void Server::launch(Slot &slot) {
     const size_t o_cfg = carve(nreq * sizeof(bf_config)), o_req = carve(nreq * sizeof(ServeReq));
     const size_t upload = off;
     const size_t d_fits = carve(rows * sizeof(bf_fit_desc)), d_order = carve(rows * 4);
-    const size_t d_info = carve(nreq * sizeof(GridInfo)), d_cur = carve(kRaggedCounters * 4);
+    const size_t d_info = carve(nreq * sizeof(GridInfo)), d_cur = carve(kRaggedCounterWords * 4);
     const size_t d_guess = carve(rows * 32);
     const size_t d_out = off;
     const size_t r_params = carve(rows * 32) - d_out, r_status = carve(rows * 4) - d_out,
@@ -331,24 +331,29 @@ void Server::launch(Slot &slot) {
             ybase = std::min(ybase, uintptr_t(r->y));
         }
     // Requests by (model, mapping class), each bucket longest first (stable
-    // in arrival order): the runs of the batch, one per bucket.
-    std::vector<std::vector<uint32_t>> bucket(4 * kRaggedClassCount);
-    for (uint32_t q = 0; q < nreq; ++q)
-        bucket[batch[q]->model * kRaggedClassCount + ragged_class_of(batch[q]->model, batch[q]->m)]
+    // in arrival order): the runs of the batch, one per bucket; two-peak
+    // requests that fix both baseline parameters have buckets of their own.
+    std::vector<std::vector<uint32_t>> bucket(5 * kRaggedClassCount);
+    for (uint32_t q = 0; q < nreq; ++q) {
+        const Request &r = *batch[q];
+        bucket[(fixed_baseline(r.model, r.cfg) ? 4u : r.model) * kRaggedClassCount +
+               ragged_class_of(r.model, r.m)]
             .push_back(q);
+    }
     auto *grids = reinterpret_cast<bf_grid_desc *>(H + o_grids);
     auto *cfgs = reinterpret_cast<bf_config *>(H + o_cfg);
     auto *reqs = reinterpret_cast<ServeReq *>(H + o_req);
     RaggedLayout layout;
     uint32_t pos = 0;
-    for (int b = 0; b < 4 * kRaggedClassCount; ++b) {
+    for (int b = 0; b < 5 * kRaggedClassCount; ++b) {
         auto &v = bucket[b];
         if (v.empty())
             continue;
         std::stable_sort(v.begin(), v.end(),
                          [&](uint32_t a, uint32_t c) { return batch[a]->m > batch[c]->m; });
-        const uint32_t model = uint32_t(b / kRaggedClassCount);
-        RaggedRun run{model, b % kRaggedClassCount, pos, 0, 0, 0, 0.0, {}};
+        const bool fixb = b >= 4 * kRaggedClassCount;
+        const uint32_t model = fixb ? uint32_t(BF_TWO_LORENTZIAN) : uint32_t(b / kRaggedClassCount);
+        RaggedRun run{model, b % kRaggedClassCount, fixb, pos, 0, 0, 0, 0.0, {}};
         for (size_t a = 0; a < v.size();) {
             const uint32_t top = batch[v[a]]->m;
             const uint32_t below = guess_slice_floor(model, top);

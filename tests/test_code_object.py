@@ -98,7 +98,9 @@ def test_fixed_baseline_two_peak_kernels_do_not_spill(resources):
     """The two-peak pool kernels that sum only the six free parameters' rows
     (baseline fixed) keep their state in registers: no private segment."""
     fixb = {k: v for k, v in resources.items()
-            if k.startswith("_ZN5bfhip14lm_pool_kernelILi3E") and k.endswith("Lb1EEEvNS_7FitArgsE")}
-    assert len(fixb) >= 5, sorted(fixb)
+            if (k.startswith("_ZN5bfhip14lm_pool_kernelILi3E") and k.endswith("Lb1EEEvNS_7FitArgsE"))
+            or (k.startswith("_ZN5bfhip21lm_ragged_pool_kernelILi3E")
+                and k.endswith("Lb1EEEvNS_10RaggedArgsE"))}
+    assert len(fixb) >= 10, sorted(fixb)
     for name, r in fixb.items():
         assert r.get("private", 0) == 0 and r.get("spill", 0) == 0, (name, r)

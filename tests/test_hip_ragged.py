@@ -66,10 +66,10 @@ def test_ragged_equals_uniform_path(hip):
 
 @pytest.mark.parametrize("shift", [0.0, 2.0e6])
 def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
-    """Doublets with the baseline fixed by equal bounds: a uniform batch runs
-    the pool kernel that sums only the six free parameters' rows (grids with
-    |x| up to 1e6) or, on a grid beyond that, the full kernel; a ragged batch
-    always runs the full kernel. The same bytes either way."""
+    """Doublets with the baseline fixed by equal bounds: the pool kernels that
+    sum only the six free parameters' rows run where every grid of the batch
+    lies within |x| <= 1e6, the full kernels elsewhere (a uniform batch on a
+    wider grid, a ragged batch holding one). The same bytes every way."""
     from batchfit_amd.ragged import fit_ragged
     x, y, _, _, _ = datasets.comparison_set("two_lorentzian", 300, 900)
     x = (shift + 80.0 * x).astype(np.float32)  # spacing 0.89, above an ulp at 2e6
@@ -82,6 +82,15 @@ def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
     b = fit_ragged([(x, y, "two_lorentzian", None, bounds)])[0]
     for u, v in zip(a, b):
         assert same(u, v), shift
+    # A ragged batch that also holds a grid beyond 1e6 runs its fixed-baseline
+    # classes in the full form.
+    xl, yl, p0l, bl, _ = datasets.comparison_set("lorentzian", 5, 400)
+    xl = (2.0e6 + 80.0 * xl).astype(np.float32)
+    bl = (np.array([0, 2.0e6 - 400, .8]), np.array([np.inf, 2.0e6 + 400, 400]))
+    c = fit_ragged([(x, y, "two_lorentzian", None, bounds),
+                    (xl, yl.astype(np.float32), "lorentzian", None, bl)])[0]
+    for u, v in zip(a, c):
+        assert same(u, v), (shift, "with a wide grid in the batch")
     # The grid is 80x the unit-scale sets', so positions and widths are too:
     # statuses and chi2 against the oracle (the unit-scale parameter gate of
     # the other tests holds this path at M = 2048 on the doublet workload).
