@@ -48,12 +48,12 @@ constexpr uint32_t kGuessLdsBytes = 65536;
 constexpr uint32_t kGuessSortedMaxPoints = 1289;
 
 // LDS floats the guess of `model` needs beyond the signal of M points: the
-// sort buffer of its edge medians and, for the two-peak model, one minimum
-// per 64-point step of the second-peak search (used after the medians, in
-// the same space).
+// sort buffer of its edge medians and, for the two-peak model, two minima per
+// 64-point step of the second-peak search (used after the medians, in the
+// same space).
 __host__ __device__ inline uint32_t guess_extra_floats(int model, uint32_t m) {
     const uint32_t sort = guess_sort_floats(m / 10 > 1 ? m / 10 : 1);
-    const uint32_t steps = model == TWO_LORENTZIAN ? (m + 63) / 64 : 0;
+    const uint32_t steps = model == TWO_LORENTZIAN ? 2 * ((m + 63) / 64) : 0;
     return sort > steps ? sort : steps;
 }
 
@@ -452,25 +452,58 @@ __device__ bool prominent_wave(const S &s, int m, int j, float h, float t, int l
     return false;
 }
 
-// One side of the prominence test of prominent_wave for the candidate at j
-// of height h, walked by its own lane over at most kLocalWalk points in
-// direction dir (±1): 1 if a point with h − v > t comes first (the side
-// passes), 2 if a point above h or the edge comes first (it fails), 0 if
-// neither lies within reach.
-constexpr int kLocalWalk = 6;
+// The prominence test of prominent_wave for the candidate at 1 ≤ j < m − 1
+// of height h, by its own lane, over the nearest kLocalWalk points of both
+// sides (the raw values they need are loaded together: one memory round trip
+// instead of one per point): 1 if each side reaches a point with h − v > t
+// before a point above h or the edge (the candidate is prominent), 2 if a
+// side reaches a point above h or the edge first (it is not), 0 if that is
+// not settled within reach.
+constexpr int kLocalWalk = 8;
 template <typename S>
-__device__ __forceinline__ int local_side(const S &s, int m, int j, float h, float t, int dir) {
-    for (int d = 1; d <= kLocalWalk; ++d) {
-        const int k = j + dir * d;
-        if (k < 0 || k >= m)
-            return 2;
-        const float v = smooth_at(s, m, k);
-        if (v > h)
-            return 2;
-        if (h - v > t)
-            return 1;
+__device__ __forceinline__ int local_sides(const S &s, int m, int j, float h, float t) {
+    constexpr int W = kLocalWalk + 2;
+    // wl[e] = s[j − e] and wr[e] = s[j + e]: the raw values around the points
+    // j − 1 … j − 8 and j + 1 … j + 8. Indices beyond an edge are clamped;
+    // such values are never used, because the side is settled at the edge.
+    float wl[W], wr[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        wl[e] = s[max(j - e, 0)];
+        wr[e] = s[min(j + e, m - 1)];
     }
-    return 0;
+    // Points in walk order; the first that settles a side decides it. Left
+    // points lie below j ≤ m − 2 and right ones above j ≥ 1, so only k = 0
+    // (left) and k = m − 1 (right) take the raw value.
+    int ls = 0, rs = 0;  // per side: 0 open, 1 passes, 2 fails
+#pragma unroll
+    for (int d = 1; d <= kLocalWalk; ++d) {
+        const int kl = j - d, kr = j + d;
+        const float vl = kl == 0 ? wl[d] : 0.25f * wl[d + 1] + 0.5f * wl[d] + 0.25f * wl[d - 1];
+        const float vr = kr == m - 1 ? wr[d] : 0.25f * wr[d - 1] + 0.5f * wr[d] + 0.25f * wr[d + 1];
+        const int sl = (kl < 0 || vl > h) ? 2 : (h - vl > t) ? 1 : 0;
+        const int sr = (kr >= m || vr > h) ? 2 : (h - vr > t) ? 1 : 0;
+        ls = ls ? ls : sl;
+        rs = rs ? rs : sr;
+    }
+    return (ls == 2 || rs == 2) ? 2 : (ls == 1 && rs == 1) ? 1 : 0;
+}
+
+// In-place suffix minima of t[0..n) by one wave: t[e] = min(t[e..n)).
+__device__ __forceinline__ void wave_suffix_min(float *t, int n, int lane) {
+    wave_sync();
+    float carry = __uint_as_float(0x7f800000u);
+    for (int b = (n - 1) & ~63; b >= 0; b -= 64) {
+        const int e = b + lane;
+        float v = e < n ? t[e] : __uint_as_float(0x7f800000u);
+        for (int d = 1; d < 64; d <<= 1)
+            v = fminf(v, __shfl_down(v, d, 64));
+        v = fminf(v, carry);
+        if (e < n)
+            t[e] = v;
+        carry = lane_value(v, 0);
+    }
+    wave_sync();
 }
 
 // Result of one fit's guess, identical in every lane of the wave.
@@ -482,9 +515,9 @@ struct GuessResult {
 
 // The guess from its edge levels on (initial_guess.hpp:170-249), by one whole
 // wave: s[j] is the baseline-removed signal (finite at every point), sfx has
-// one float per 2^sfx_shift 64-point steps of the second-peak search (the
-// two-peak model; ((m − 2) / 64 >> sfx_shift) + 1 of them), res holds the
-// baseline and GS_INVALID on entry.
+// two floats per run of 2^sfx_shift 64-point steps (the two-peak model; runs
+// = ((m − 2) / 64 >> sfx_shift) + 1 runs), res holds the baseline and
+// GS_INVALID on entry.
 template <int MODEL, typename S>
 __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const float *x, int m,
                                   const GridInfo &grid, float left_level, float slope, float noise,
@@ -499,12 +532,21 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
     // smoothed value is formed once, in the lane of its point; a point's
     // neighbours come from the adjacent lanes, and across a step's edges
     // from the previous and next steps.
+    // The two-peak model's scan also records, per run of 2^sfx_shift steps
+    // (the last step holds the point m − 1 alone when (m − 2) % 64 = 0), the
+    // minimum of the smoothed signal within the run (sfx[run], made suffix
+    // minima below) and from the start through the run (pfx[run]).
+    const int runs = paired ? ((m - 2) / 64 >> sfx_shift) + 1 : 0;
+    float *const pfx = sfx + runs;
     float best = 0.0f;
     int first = 0x7fffffff;
     {
+        const float inf = __uint_as_float(0x7f800000u);
+        const int run_mask = (1 << sfx_shift) - 1;
         float cur = smooth_in(s, m, 1 + lane);
         float before = s[0];  // the smoothed value at 0 is the raw one
-        for (int base = 1; base + 1 < m; base += 64) {
+        float prefix = s[0], run = inf;
+        for (int base = 1; base < (paired ? m : m - 1); base += 64) {
             const int j = base + lane;
             const float next = smooth_in(s, m, j + 64);
             const float left = from_lane_below(cur, before);
@@ -512,6 +554,19 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
             if (j + 1 < m && cur > best && cur > left && cur >= right) {
                 best = cur;
                 first = j;
+            }
+            if constexpr (paired) {
+                const float step_min = wave_min_fx(j < m ? cur : inf);
+                prefix = fminf(prefix, step_min);
+                run = fminf(run, step_min);
+                const int c = (base - 1) / 64;
+                if (((c + 1) & run_mask) == 0 || base + 64 >= m) {
+                    if (lane == 0) {
+                        sfx[c >> sfx_shift] = run;
+                        pfx[c >> sfx_shift] = prefix;
+                    }
+                    run = inf;
+                }
             }
             before = lane_value(cur, 63);
             cur = next;
@@ -546,23 +601,11 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
         // increasing index, so a later one must be strictly larger to win.
         const float t = 3.0f * noise;
         // A side can pass only if it holds a point with h − v > t, so only if
-        // h − (minimum over any range that contains the side) > t. The
-        // minima from the first 64-point step of a run of 2^sfx_shift steps
-        // to the end (sfx[run]) and from the start through the current step
-        // (pmin) reject most candidates without a walk.
-        {
-            const int run_mask = (1 << sfx_shift) - 1;
-            float running = __uint_as_float(0x7f800000u);
-            for (int c = (m - 2) / 64; c >= 0; --c) {
-                const int j = 1 + 64 * c + lane;
-                running = fminf(running, wave_min_fx(j < m ? smooth_at(s, m, j)
-                                                           : __uint_as_float(0x7f800000u)));
-                if (lane == 0 && (c & run_mask) == 0)
-                    sfx[c >> sfx_shift] = running;
-            }
-            wave_sync();
-        }
-        float pmin = s[0];
+        // h − (minimum over any range that contains the side) > t: the minima
+        // from the start of a candidate's run to the end (sfx[run]) and from
+        // the start through its run (pfx[run]) reject most candidates
+        // without a walk.
+        wave_suffix_min(sfx, runs, lane);
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
         float cur = smooth_in(s, m, 1 + lane);
@@ -576,25 +619,21 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
             before = lane_value(cur, 63);
             cur = next;
             bool cand = false;
-            pmin = fminf(pmin, wave_min_fx(j + 1 < m ? v : __uint_as_float(0x7f800000u)));
-            const float right_min = sfx[((base - 1) / 64) >> sfx_shift];
+            const int r = ((base - 1) / 64) >> sfx_shift;
+            const float left_min = pfx[r], right_min = sfx[r];
             if (j + 1 < m) {
-                cand = v > v_best && v - pmin > t && v - right_min > t && v > left && v >= right;
+                cand = v > v_best && v - left_min > t && v - right_min > t && v > left &&
+                       v >= right;
                 if (cand) {
                     const int idx = raw_peak_dev(s, j);
                     const int bins = idx > first ? idx - first : first - idx;
                     cand = !(bins < 3 || fabsf(x[idx] - primary.center) < separation);
                 }
             }
-            // Each candidate first walks the nearest kLocalWalk points of both
-            // sides on its own lane (most are settled there, by a higher
-            // point close by); the rest are walked by the whole wave.
-            int state = 0;  // 1: prominent, 2: not, 0: not settled
-            if (cand) {
-                const int ls = local_side(s, m, j, v, t, -1);
-                const int rs = ls == 2 ? 2 : local_side(s, m, j, v, t, 1);
-                state = (ls == 2 || rs == 2) ? 2 : (ls == 1 && rs == 1) ? 1 : 0;
-            }
+            // Each candidate first reads the nearest kLocalWalk points of
+            // both sides on its own lane (most are settled there); the rest
+            // are walked by the whole wave.
+            const int state = cand ? local_sides(s, m, j, v, t) : 0;
             const uint64_t settled_yes = __ballot(cand && state == 1);
             const uint64_t settled_no = __ballot(cand && state == 2);
             // This step's candidates in increasing index; one that is not

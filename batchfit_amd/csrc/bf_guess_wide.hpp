@@ -28,8 +28,8 @@ struct MemSignal {
     }
 };
 
-// Suffix minima per wave of the wide guess (floats): runs of 2^k steps keep
-// ((M − 2) / 64 >> k) + 1 of them within this table.
+// Run minima per wave of the wide guess (floats): runs of 2^k steps keep
+// two per run, ((M − 2) / 64 >> k) + 1 runs, within this table.
 constexpr int kWideSuffix = 1024;
 constexpr int kWideWaves = 4;  // waves (fits) per block
 
@@ -66,7 +66,7 @@ __device__ GuessResult guess_core_wide(const float *y, const float *x, int m, co
     if (wave_or_i(bad) || !finite_f(noise))
         return res;
     int shift = 0;
-    while (((m - 2) / 64 >> shift) >= kWideSuffix)
+    while (((m - 2) / 64 >> shift) >= kWideSuffix / 2)
         ++shift;
     return guess_tail<MODEL>(sig, sfx, shift, x, m, grid, lv.left, slope, noise, lane, res);
 }
