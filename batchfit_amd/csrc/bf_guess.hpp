@@ -414,6 +414,60 @@ template <typename S> __device__ __forceinline__ int raw_peak_dev(const S &s, in
     return best;
 }
 
+// Length of the prefix of [0, m) on which pred holds (pred true up to some
+// index and false from there on), found by the whole wave from index 0 ≤
+// from < m outwards, 64 indices per step: the boundary lies near `from`.
+template <typename Pred>
+__device__ __forceinline__ int prefix_length(int m, int from, Pred pred, int lane) {
+    if (pred(from)) {
+        for (int b = from + 1; b < m; b += 64) {
+            const int i = b + lane;
+            const uint64_t no = __ballot(i < m && !pred(i));
+            if (no)
+                return b + __builtin_ctzll(no);
+        }
+        return m;
+    }
+    for (int b = from - 1; b >= 0; b -= 64) {
+        const int i = b - lane;
+        const uint64_t yes = __ballot(i >= 0 && pred(i));
+        if (yes)
+            return b - __builtin_ctzll(yes) + 1;
+    }
+    return 0;
+}
+
+// The raw values at j − 1, j, j + 1 where smooth_in reads them (zeros
+// elsewhere) and the smoothed value v = smooth_in(s, m, j).
+struct Raw3 {
+    float a, b, c, v;
+};
+template <typename S> __device__ __forceinline__ Raw3 raw3_in(const S &s, int m, int j) {
+    Raw3 r{0.0f, 0.0f, 0.0f, 0.0f};
+    if (j + 1 < m) {
+        r.a = s[j - 1];
+        r.b = s[j];
+        r.c = s[j + 1];
+        r.v = 0.25f * r.a + 0.5f * r.b + 0.25f * r.c;
+    } else if (j < m) {
+        r.b = s[j];
+        r.v = r.b;
+    }
+    return r;
+}
+// raw_peak_dev(s, j) from the raw values of raw3_in(s, m, j), j + 1 < m.
+__device__ __forceinline__ int raw_peak3(const Raw3 &r, int j) {
+    int best = j - 1;
+    float bv = r.a;
+    if (bv < r.b) {
+        best = j;
+        bv = r.b;
+    }
+    if (bv < r.c)
+        best = j + 1;
+    return best;
+}
+
 // Whether the smoothed maximum at j of height h has prominence > t
 // (initial_guess.hpp:38-54 and the `<= 3·noise` test of :212), decided by
 // the whole wave for one candidate (j, h, t wave-uniform). The reference
@@ -600,6 +654,15 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
         // ties (initial_guess.hpp:203-217): candidates are taken in
         // increasing index, so a later one must be strictly larger to win.
         const float t = 3.0f * noise;
+        // x is strictly increasing (guess_grid_ok), so x[i] − c is
+        // non-decreasing in i and |x[i] − c| < separation holds exactly on
+        // the indices [sep_lo, sep_hi): the prefixes of the points with
+        // x[i] − c ≤ −separation and with x[i] − c < separation (a NaN
+        // centre or separation gives sep_lo = m: no index).
+        const float c = primary.center;
+        const int sep_lo =
+            prefix_length(m, first, [&](int i) { return !(x[i] - c > -separation); }, lane);
+        const int sep_hi = prefix_length(m, first, [&](int i) { return x[i] - c < separation; }, lane);
         // A side can pass only if it holds a point with h − v > t, so only if
         // h − (minimum over any range that contains the side) > t: the minima
         // from the start of a candidate's run to the end (sfx[run]) and from
@@ -608,16 +671,16 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
         wave_suffix_min(sfx, runs, lane);
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
-        float cur = smooth_in(s, m, 1 + lane);
+        Raw3 cur = raw3_in(s, m, 1 + lane);
         float before = s[0];
         for (int base = 1; base + 1 < m; base += 64) {
             const int j = base + lane;
-            const float v = cur;  // the smoothed value wherever j + 1 < m
-            const float next = smooth_in(s, m, j + 64);
-            const float left = from_lane_below(cur, before);
-            const float right = from_lane_above(cur, lane_value(next, 0));
-            before = lane_value(cur, 63);
-            cur = next;
+            const Raw3 here = cur;
+            const float v = here.v;  // the smoothed value wherever j + 1 < m
+            cur = raw3_in(s, m, j + 64);
+            const float left = from_lane_below(v, before);
+            const float right = from_lane_above(v, lane_value(cur.v, 0));
+            before = lane_value(v, 63);
             bool cand = false;
             const int r = ((base - 1) / 64) >> sfx_shift;
             const float left_min = pfx[r], right_min = sfx[r];
@@ -625,9 +688,9 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
                 cand = v > v_best && v - left_min > t && v - right_min > t && v > left &&
                        v >= right;
                 if (cand) {
-                    const int idx = raw_peak_dev(s, j);
+                    const int idx = raw_peak3(here, j);
                     const int bins = idx > first ? idx - first : first - idx;
-                    cand = !(bins < 3 || fabsf(x[idx] - primary.center) < separation);
+                    cand = !(bins < 3 || (idx >= sep_lo && idx < sep_hi));
                 }
             }
             // Each candidate first reads the nearest kLocalWalk points of

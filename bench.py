@@ -195,7 +195,10 @@ def bench_metric(S, args):
             "hbm_peak_GBs": PEAK_HBM_GBS,
             "note": "peak is the published FP32 figure, which counts packed (dual-issue) FP32; "
                     "packed FP32 is disabled in this build, so 78.6 TFLOP/s is the scalar-issue "
-                    "ceiling; exp/rcp are quarter-rate transcendentals",
+                    "ceiling at 2.4 GHz; the kernel is VALU-issue bound (about 41 VALU "
+                    "instructions per point and pass, whose issue at 4 cycles per wave64 "
+                    "instruction on 1024 SIMDs at 2.4 GHz takes 0.93 of its time) and the card "
+                    "runs below 2.4 GHz under this load (power limit)",
         },
         "outcome": {"status_counts": np.bincount(status, minlength=6).tolist(),
                     "mean_iterations": float(iterations.mean())},
