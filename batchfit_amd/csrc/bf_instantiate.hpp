@@ -60,8 +60,16 @@ hipError_t go(const Plan &, const FitArgs &args, hipStream_t stream) {
                 hipError_t e = hipGetLastError();
                 if (e != hipSuccess || !args.cov)
                     return e;
+                if constexpr (MODEL == TWO_LORENTZIAN) {
+                    if (full.wide_grid_only) {
+                        hipLaunchKernelGGL((lm_cov_kernel<MODEL, G, MIXED, true>), dim3(blocks),
+                                           dim3(256), 0, stream, args);
+                        if ((e = hipGetLastError()) != hipSuccess)
+                            return e;
+                    }
+                }
                 hipLaunchKernelGGL((lm_cov_kernel<MODEL, G, MIXED>), dim3(blocks), dim3(256), 0,
-                                   stream, args);
+                                   stream, full);
                 return hipGetLastError();
             }
         }

@@ -500,16 +500,19 @@ __device__ __forceinline__ void fused_pass(const Points<G, R, stream_chunk(MODEL
 
 // JᵀJ only (uncertainty at the accepted point), always compensated per lane
 // (cpu/uncertainty.hpp:297-309); `jac_ok` is false if any Jacobian entry is
-// non-finite.
-template <int MODEL, int G, int R, typename S>
+// non-finite. Only the rows of the first PA parameters are summed (the lower
+// triangle's first PA(PA+1)/2 terms); the others are written as zeros, for a
+// caller that replaces them (fixed parameters). Each term's sum is formed the
+// same way whatever PA is, so the summed terms keep their bits.
+template <int MODEL, int G, int R, typename S, int PA = Params<MODEL>::P>
 __device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk(MODEL)> &pts, const float *p,
                                                  S *info, bool &jac_ok) {
-    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
+    constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI, TA = PA * (PA + 1) / 2;
     ModelConsts<MODEL> mc;
     mc.set(p);
-    Acc<true> acc[TRI];
+    Acc<true> acc[TA];
 #pragma unroll
-    for (int t = 0; t < TRI; ++t)
+    for (int t = 0; t < TA; ++t)
         acc[t].zero();
     int bad = 0;
     pts.each([&](float xj, float, auto first) {
@@ -518,32 +521,37 @@ __device__ __forceinline__ void information_pass(const Points<G, R, stream_chunk
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             bad |= !finite_f(jac[k]);
+            if (k < PA) {
 #pragma unroll
-            for (int l = 0; l <= k; ++l)
-                acc[tri(k, l)].add_prod(jac[k], jac[l], first);
+                for (int l = 0; l <= k; ++l)
+                    acc[tri(k, l)].add_prod(jac[k], jac[l], first);
+            }
         }
     }, [&](auto kind) {
 #pragma unroll
-        for (int t = 0; t < TRI; ++t)
+        for (int t = 0; t < TA; ++t)
             acc[t].flush(kind);
     });
     if constexpr (G > 1)
         bad = group_or<G>(bad);
     jac_ok = !bad;
     if constexpr (sizeof(S) == 8) {
-        double v[TRI];
+        double v[TA];
 #pragma unroll
-        for (int t = 0; t < TRI; ++t)
+        for (int t = 0; t < TA; ++t)
             v[t] = double(acc[t].hi()) - double(acc[t].lo());
-        group_allreduce_d<G, TRI>(v, pts.gl);
+        group_allreduce_d<G, TA>(v, pts.gl);
 #pragma unroll
-        for (int t = 0; t < TRI; ++t)
+        for (int t = 0; t < TA; ++t)
             info[t] = v[t] * double(term_scale<MODEL>(t));
     } else {
 #pragma unroll
-        for (int t = 0; t < TRI; ++t)
+        for (int t = 0; t < TA; ++t)
             info[t] = reduce_term<G, S, true>(acc[t]) * term_scale<MODEL>(t);
     }
+#pragma unroll
+    for (int t = TA; t < TRI; ++t)
+        info[t] = S(0);
 }
 
 template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const float *p,
@@ -558,8 +566,13 @@ template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const fl
 }
 
 // Covariance at the accepted point (cpu/uncertainty.hpp:13-110), written for
-// one fit by lane 0 of its group.
-template <int MODEL, int G, int R, typename S>
+// one fit by lane 0 of its group. FIXB (the two-peak model with both
+// baseline parameters fixed, on a grid with |x| ≤ kFixedBaselineMaxX): the
+// information pass sums only the six free parameters' rows; the baseline
+// rows are replaced by the identity's below in either form, and on such a
+// grid their sums are finite whenever the free rows' are (Cauchy–Schwarz),
+// so no finiteness test changes.
+template <int MODEL, int G, int R, typename S, bool FIXB = false>
 __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, stream_chunk(MODEL)> &pts, uint32_t fit,
                                            int gl, const float *p, int32_t status, float chi2) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
@@ -597,7 +610,7 @@ __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, 
         return fail(US_INVALID);
     S info[TRI];
     bool jac_ok;
-    information_pass<MODEL, G, R, S>(pts, p, info, jac_ok);
+    information_pass<MODEL, G, R, S, (FIXB && MODEL == TWO_LORENTZIAN) ? 6 : P>(pts, p, info, jac_ok);
     if (!jac_ok)
         return fail(US_INVALID);
 #pragma unroll
@@ -1222,13 +1235,16 @@ __global__ void __launch_bounds__(256, lm_min_waves(MODEL, G, 0)) lm_pool_kernel
 // Uncertainties after a retiring launch: one group per fit, from the fit's
 // written parameters, status and χ² (the values solve_fit hands to
 // covariance), over the streamed points of the same group size.
-template <int MODEL, int G, bool MIXED>
+// FIXB: the fixed-baseline form, launched before the full one with
+// wide_grid_only set; exactly one of them runs, by the grid's largest |x|.
+template <int MODEL, int G, bool MIXED, bool FIXB = false>
 __global__ void __launch_bounds__(256) lm_cov_kernel(FitArgs a) {
     using S = typename std::conditional<MIXED, double, float>::type;
     constexpr int P = Params<MODEL>::P;
     const uint32_t fit = blockIdx.x * (256 / G) + threadIdx.x / G;
     const int gl = int(threadIdx.x % G);
-    if (fit >= a.n)
+    const bool narrow = a.grid->max_abs_x <= kFixedBaselineMaxX;
+    if (fit >= a.n || (FIXB ? !narrow : (a.wide_grid_only && narrow)))
         return;
     float p[P];
 #pragma unroll
@@ -1236,7 +1252,7 @@ __global__ void __launch_bounds__(256) lm_cov_kernel(FitArgs a) {
         p[k] = a.params[size_t(fit) * P + k];
     Points<G, 0, stream_chunk(MODEL)> pts;
     pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
-    covariance<MODEL, G, 0, S>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit]);
+    covariance<MODEL, G, 0, S, FIXB>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit]);
 }
 
 // Workgroup per fit (G = 128 or 256 lanes, 2 or 4 waves): every workgroup

@@ -217,14 +217,15 @@ __global__ void __launch_bounds__(256, lm_ragged_min_waves(MODEL, G, 0))
 }
 
 // Uncertainties of a streamed class after its retiring launch: one group per
-// fit, from the fit's written parameters, status and χ².
-template <int MODEL, int G, bool MIXED>
+// fit, from the fit's written parameters, status and χ². FIXB and wide_only
+// as for lm_ragged_pool_kernel: exactly one of the two forms runs.
+template <int MODEL, int G, bool MIXED, bool FIXB = false>
 __global__ void __launch_bounds__(256) lm_ragged_cov_kernel(RaggedArgs r) {
     using S = typename std::conditional<MIXED, double, float>::type;
     constexpr int P = Params<MODEL>::P;
     const uint32_t task = blockIdx.x * (256 / G) + threadIdx.x / G;
     const int gl = int(threadIdx.x % G);
-    if (task >= r.count)
+    if (task >= r.count || (FIXB ? !*r.narrow : (r.wide_only && *r.narrow)))
         return;
     bf_fit_desc d = r.fits[r.order[task]];
     if constexpr (G == 64)
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(256) lm_ragged_cov_kernel(RaggedArgs r) {
         p[k] = a.params[k];
     Points<G, 0, stream_chunk(MODEL)> pts;
     pts.load(a.x, a.y, 1, gl, int(a.m));
-    covariance<MODEL, G, 0, S>(a, pts, 0, gl, p, a.status[0], a.chi2[0]);
+    covariance<MODEL, G, 0, S, FIXB>(a, pts, 0, gl, p, a.status[0], a.chi2[0]);
 }
 
 // Data-only guess of one class of a ragged batch, one wave per fit; LDS sized

@@ -103,9 +103,17 @@ hipError_t ragged_go(const RaggedArgs &args, uint32_t blocks, hipStream_t stream
         if (e != hipSuccess || !args.cov)
             return e;
         constexpr uint32_t per_block = 256 / G;
-        hipLaunchKernelGGL((lm_ragged_cov_kernel<MODEL, G, MIXED>),
-                           dim3((args.count + per_block - 1) / per_block), dim3(256), 0, stream,
-                           args);
+        const dim3 cov_blocks((args.count + per_block - 1) / per_block);
+        if constexpr (MODEL == TWO_LORENTZIAN) {
+            if (args.fixed_baseline) {
+                hipLaunchKernelGGL((lm_ragged_cov_kernel<MODEL, G, MIXED, true>), cov_blocks,
+                                   dim3(256), 0, stream, args);
+                if ((e = hipGetLastError()) != hipSuccess)
+                    return e;
+            }
+        }
+        hipLaunchKernelGGL((lm_ragged_cov_kernel<MODEL, G, MIXED>), cov_blocks, dim3(256), 0,
+                           stream, full);
         return hipGetLastError();
     }
     hipLaunchKernelGGL((lm_ragged_kernel<MODEL, G, R, MIXED, MIXED>), dim3(blocks), dim3(256), 0,

@@ -83,7 +83,7 @@ def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
     for u, v in zip(a, b):
         assert same(u, v), shift
     # A ragged batch that also holds a grid beyond 1e6 runs its fixed-baseline
-    # classes in the full form.
+    # classes in the full form (the LM and the uncertainty kernels).
     xl, yl, p0l, bl, _ = datasets.comparison_set("lorentzian", 5, 400)
     xl = (2.0e6 + 80.0 * xl).astype(np.float32)
     bl = (np.array([0, 2.0e6 - 400, .8]), np.array([np.inf, 2.0e6 + 400, 400]))
@@ -91,6 +91,13 @@ def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
                     (xl, yl.astype(np.float32), "lorentzian", None, bl)])[0]
     for u, v in zip(a, c):
         assert same(u, v), (shift, "with a wide grid in the batch")
+    au = hip.fit(x, y, "two_lorentzian", None, bounds, "hip", uncertainties=True)
+    bu = fit_ragged([(x, y, "two_lorentzian", None, bounds)], uncertainties=True)[0]
+    cu = fit_ragged([(x, y, "two_lorentzian", None, bounds),
+                     (xl, yl.astype(np.float32), "lorentzian", None, bl)], uncertainties=True)[0]
+    for u, v, w in zip(au, bu, cu):
+        assert same(u, v) and same(u, w), (shift, "uncertainties")
+    assert np.sum(au.uncertainty_status == 0) > 100, shift
     # The grid is 80x the unit-scale sets', so positions and widths are too:
     # statuses and chi2 against the oracle (the unit-scale parameter gate of
     # the other tests holds this path at M = 2048 on the doublet workload).
