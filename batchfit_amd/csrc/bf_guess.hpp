@@ -2,8 +2,8 @@
 // fit (cpu/initial_guess.hpp:120-249).
 //
 // The fit's signal is staged in LDS; the edge medians are exact order
-// statistics selected in registers (an LDS sort for windows beyond 1024
-// values); every scan of the reference (smoothed-maximum search, half-height
+// statistics selected in registers (searched by value over the staged
+// signal for windows beyond 1024 values); every scan of the reference (smoothed-maximum search, half-height
 // walks, prominence walks, valley search) becomes a wave-wide reduction with
 // the reference's tie rule (first index wins). The O(1) arithmetic between
 // scans runs redundantly on all lanes with FP contraction disabled, so the
@@ -21,45 +21,45 @@ struct GuessArgs {
     const GridInfo *grid;
     uint32_t n, m;
     uint32_t m_pad;   // LDS floats reserved for the signal
-    uint32_t sort_n;  // LDS floats reserved beyond the signal (guess_extra_floats)
+    uint32_t extra_n;  // LDS floats reserved beyond the signal (guess_extra_floats)
     float *params;    // [N,P]
     float *baseline;  // [N] or null
     int32_t *gstatus; // [N] or null
 };
 
-// LDS floats the guess needs beyond the signal for an edge window of `count`
-// points: medians of up to 1024 values are selected in registers, so only
-// windows whose union exceeds that (M > 5129) sort in LDS.
-__host__ __device__ inline uint32_t guess_sort_floats(uint32_t count) {
-    if (2 * count <= 1024)
-        return 0;
-    uint32_t n = 1;
-    while (n < 2 * count)
-        n <<= 1;
-    return n;
-}
-
-// LDS the one-wave guess may take per fit (the signal and the sort buffer of
-// its edge medians, or its suffix minima): M up to 12,288 points. Longer
-// spectra take the wide guess (bf_guess_wide.hpp), which stages nothing.
+// LDS the one-wave guess may take per fit (the signal and, for the two-peak
+// model, its step-minimum tables): M up to 16,384 points (15,886 for the
+// two-peak model). Longer spectra take the wide guess (bf_guess_wide.hpp),
+// which stages nothing.
 constexpr uint32_t kGuessLdsBytes = 65536;
 // Largest M whose edge windows (M / 10 values each) the sorting-network guess
 // of the single-peak models holds: 128 values, eight lanes of 16 slots.
 constexpr uint32_t kGuessSortedMaxPoints = 1289;
+// Largest M for which the automatic choice takes the one-wave kernel where
+// the wide kernel also applies (fit-major rows). Beyond it the wide kernel
+// is faster: from 5,130 points both search the edge levels by value, and
+// without LDS the wide kernel runs more waves per CU (8,192 fits at M = 8192:
+// two-peak 1.9 against 2.6 ms, Lorentzian 0.70 against 1.13 ms; at M = 4096
+// the one-wave kernel is ahead, 0.56 against 0.89 and 0.16 against 0.30 ms).
+constexpr uint32_t kGuessWaveMaxPoints = 5120;
 
-// LDS floats the guess of `model` needs beyond the signal of M points: the
-// sort buffer of its edge medians and, for the two-peak model, two minima per
-// 64-point step of the second-peak search (used after the medians, in the
-// same space).
+// LDS floats the guess of `model` needs beyond the signal of M points: for
+// the two-peak model, two minima per 64-point step of the second-peak search.
 __host__ __device__ inline uint32_t guess_extra_floats(int model, uint32_t m) {
-    const uint32_t sort = guess_sort_floats(m / 10 > 1 ? m / 10 : 1);
-    const uint32_t steps = model == TWO_LORENTZIAN ? 2 * ((m + 63) / 64) : 0;
-    return sort > steps ? sort : steps;
+    return model == TWO_LORENTZIAN ? 2 * ((m + 63) / 64) : 0;
 }
 
 // Whether the one-wave guess of `model` holds M points in its LDS.
 __host__ __device__ inline bool guess_fits_lds(int model, uint32_t m) {
     return (uint64_t(m) + guess_extra_floats(model, m)) * sizeof(float) <= kGuessLdsBytes;
+}
+
+// The largest M for which guess_fits_lds(model, M) holds.
+__host__ __device__ inline uint32_t guess_lds_max_points(int model) {
+    uint32_t m = kGuessLdsBytes / sizeof(float);
+    while (m > 0 && !guess_fits_lds(model, m))
+        --m;
+    return m;
 }
 
 // Which guess kernel serves a launch: the fastest that applies (AUTO), or,
@@ -137,14 +137,17 @@ __device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
     return max(uint32_t(__builtin_amdgcn_readlane(int(v), 0)),
                uint32_t(__builtin_amdgcn_readlane(int(v), 32)));
 }
-// Exact wave minimum of floats with no NaN (DPP and swizzle steps).
+// Exact wave minimum of floats with no NaN, as a wave-uniform value: the
+// all-reduce steps of xlane within each 32-lane half (DPP within rows, one
+// swizzle), then lanes 0 and 32.
 __device__ __forceinline__ float wave_min_fx(float v) {
-    v = fminf(v, __int_as_float(xor_lane<1>(__float_as_int(v))));
-    v = fminf(v, __int_as_float(xor_lane<2>(__float_as_int(v))));
-    v = fminf(v, __int_as_float(xor_lane<4>(__float_as_int(v))));
-    v = fminf(v, __int_as_float(xor_lane<8>(__float_as_int(v))));
-    v = fminf(v, __int_as_float(xor_lane<16>(__float_as_int(v))));
-    return fminf(v, __int_as_float(xor_lane<32>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xlane<1>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xlane<2>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xlane<4>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xlane<8>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(xlane<16>(__float_as_int(v))));
+    return fminf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
 }
 __device__ __forceinline__ float wave_min_f(float v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -173,32 +176,6 @@ __device__ __forceinline__ void wave_argmin(float &v, int &idx) {
             idx = wi;
         }
     }
-}
-
-// Ascending bitonic sort of buf[0..n), n a power of two, padded by +inf.
-__device__ void wave_sort(float *buf, int n, int lane) {
-    for (int k = 2; k <= n; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < n; i += 64) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const float a = buf[i], b = buf[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        buf[i] = b;
-                        buf[ixj] = a;
-                    }
-                }
-            }
-            wave_sync();
-        }
-}
-
-__device__ __forceinline__ int pow2_at_least(int v) {
-    int n = 1;
-    while (n < v)
-        n <<= 1;
-    return n;
 }
 
 // Order-preserving 32-bit key of a finite float (-0 sorts just below +0) and
@@ -279,10 +256,9 @@ __device__ float select_median(int count, int lane, F &&get) {
     return std_midpoint(key_float(lower), key_float(t));
 }
 
-// Median of get(0..count) (initial_guess.hpp:25-30): exact selection in
-// registers up to 1024 values, an LDS sort in buf beyond.
-template <typename F>
-__device__ float wave_median(float *buf, int count, int lane, F &&get) {
+// Median of get(0..count), count ≤ 1024 (initial_guess.hpp:25-30): exact
+// selection in registers.
+template <typename F> __device__ float wave_median(int count, int lane, F &&get) {
     if (count <= 64)
         return select_median<1>(count, lane, get);
     if (count <= 128)
@@ -291,18 +267,17 @@ __device__ float wave_median(float *buf, int count, int lane, F &&get) {
         return select_median<4>(count, lane, get);
     if (count <= 512)
         return select_median<8>(count, lane, get);
-    if (count <= 1024)
-        return select_median<16>(count, lane, get);
-    const int n = pow2_at_least(count);
-    for (int i = lane; i < n; i += 64)
-        buf[i] = i < count ? get(i) : __uint_as_float(0x7f800000u);
-    wave_sync();
-    wave_sort(buf, n, lane);
-    const float r = count % 2 ? buf[count / 2] : std_midpoint(buf[count / 2 - 1], buf[count / 2]);
-    wave_sync();
-    return r;
+    return select_median<16>(count, lane, get);
 }
 
+// The four edge levels of a row (initial_guess.hpp:142-165): the median of
+// each edge window, of both, and of the absolute deviations from each edge's
+// own median. stream_levels (bf_guess_group.hpp) reads them from y[0..m) by
+// one whole wave at every step of its searches, for windows of any length.
+struct EdgeLevels {
+    float left, right, both, spread;
+};
+__device__ EdgeLevels stream_levels(const float *y, int m, int count, int lane);
 
 struct GuessPeak {
     float amplitude, center, width, left_radius, right_radius;
@@ -758,9 +733,10 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
 
 // The guess of one fit by one whole wave (lane = lane in the wave), on the
 // wave's own LDS: s[0..m) holds the fit's raw observations on entry (it is
-// turned into the baseline-removed signal), buf has the sort floats the host
-// reserved (guess_extra_floats), y_bad says whether any observation is
-// non-finite (initial_guess.hpp:131-139).
+// turned into the baseline-removed signal), buf has the floats the host
+// reserved beyond it (guess_extra_floats: the two-peak model's step-minimum
+// tables), y_bad says whether any observation is non-finite
+// (initial_guess.hpp:131-139).
 template <int MODEL>
 __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, GridInfo grid,
                                   bool y_bad, int lane) {
@@ -775,16 +751,27 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
     if (m < 5 || !grid.guess_grid_ok || y_bad)
         return finish();
     const int count = grid.count;
-    // Edge levels, baseline and noise scale (initial_guess.hpp:142-169).
-    const float left_level = wave_median(buf, count, lane, [&](int i) { return s[i]; });
-    const float right_level =
-        wave_median(buf, count, lane, [&](int i) { return s[m - count + i]; });
-    baseline = wave_median(buf, 2 * count, lane, [&](int i) {
-        return i < count ? s[i] : s[m - count + (i - count)];
-    });
-    const float spread = wave_median(buf, 2 * count, lane, [&](int i) {
-        return i < count ? fabsf(s[i] - left_level) : fabsf(s[m - count + (i - count)] - right_level);
-    });
+    // Edge levels, baseline and noise scale (initial_guess.hpp:142-169):
+    // selected in registers while both windows together hold at most 1024
+    // values (M < 5,130), else searched by value over the staged signal.
+    float left_level, right_level, spread;
+    if (2 * count <= 1024) {
+        left_level = wave_median(count, lane, [&](int i) { return s[i]; });
+        right_level = wave_median(count, lane, [&](int i) { return s[m - count + i]; });
+        baseline = wave_median(2 * count, lane, [&](int i) {
+            return i < count ? s[i] : s[m - count + (i - count)];
+        });
+        spread = wave_median(2 * count, lane, [&](int i) {
+            return i < count ? fabsf(s[i] - left_level)
+                             : fabsf(s[m - count + (i - count)] - right_level);
+        });
+    } else {
+        const EdgeLevels lv = stream_levels(s, m, count, lane);
+        left_level = lv.left;
+        right_level = lv.right;
+        baseline = lv.both;
+        spread = lv.spread;
+    }
     const float left_x = grid.left_x, right_x = grid.right_x;
     const float slope = paired ? (right_level - left_level) / (right_x - left_x) : 0.0f;
     const float level = paired ? left_level : baseline;
@@ -818,7 +805,7 @@ __device__ GuessResult guess_core(float *s, float *buf, const float *x, int m, G
 }
 
 // The guess of fit `fit` of `a`, run by one whole wave that owns the dynamic
-// LDS of its block (a.m_pad + a.sort_n floats).
+// LDS of its block (a.m_pad + a.extra_n floats).
 template <int MODEL> __device__ void guess_fit(const GuessArgs &a, uint32_t fit) {
     constexpr int P = Params<MODEL>::P;
     extern __shared__ float lds[];

@@ -48,7 +48,9 @@ BF_DECLARE_GROUPS(3)
         hipError_t e = hipSuccess;                                                             \
         if (form == GUESS_AUTO && launch_guess_group<M>(args, stream, e))                      \
             return e;                                                                          \
-        if (form == GUESS_WIDE || (form == GUESS_AUTO && lds_bytes > kGuessLdsBytes)) {        \
+        if (form == GUESS_WIDE ||                                                              \
+            (form == GUESS_AUTO && (lds_bytes > kGuessLdsBytes ||                              \
+                                    (args.m > kGuessWaveMaxPoints && args.ys_pt == 1)))) {     \
             if (args.ys_pt != 1)                                                               \
                 return hipErrorInvalidValue;                                                   \
             hipLaunchKernelGGL((guess_wide_kernel<M>), dim3((args.n + kWideWaves - 1) /        \

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) lm_ragged_cov_kernel(RaggedArgs r) {
 template <int MODEL> __global__ void __launch_bounds__(64) guess_ragged_kernel(
     const bf_fit_desc *fits, const uint32_t *order, uint32_t count, const bf_grid_desc *grids,
     const GridInfo *grid_info, const float *x, const float *y, float *guess, uint32_t m_pad,
-    uint32_t sort_n) {
+    uint32_t extra_n) {
     const uint32_t task = blockIdx.x;
     if (task >= count)
         return;
@@ -261,7 +261,7 @@ template <int MODEL> __global__ void __launch_bounds__(64) guess_ragged_kernel(
     g.n = 1;
     g.m = d.points;
     g.m_pad = m_pad;
-    g.sort_n = sort_n;
+    g.extra_n = extra_n;
     g.params = guess + size_t(d.out_row) * 8;
     g.baseline = nullptr;
     g.gstatus = nullptr;
@@ -416,6 +416,6 @@ template <int MODEL>
 hipError_t launch_ragged_guess(const bf_fit_desc *fits, const uint32_t *order, uint32_t count,
                                const bf_grid_desc *grids, const GridInfo *grid_info,
                                const float *x, const float *y, float *guess, uint32_t m_pad,
-                               uint32_t sort_n, hipStream_t stream);
+                               uint32_t extra_n, hipStream_t stream);
 
 }  // namespace bfhip

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) guess_ragged_single_kernel(
     g.n = 1;
     g.m = d.points;
     g.m_pad = 0;
-    g.sort_n = 0;
+    g.extra_n = 0;
     g.params = guess + size_t(d.out_row) * 8;
     g.baseline = nullptr;
     g.gstatus = nullptr;
@@ -268,7 +268,7 @@ template <>
 hipError_t launch_ragged_guess<BF_MODEL>(const bf_fit_desc *fits, const uint32_t *order,
                                          uint32_t count, const bf_grid_desc *grids,
                                          const GridInfo *grid_info, const float *x, const float *y,
-                                         float *guess, uint32_t m_pad, uint32_t sort_n,
+                                         float *guess, uint32_t m_pad, uint32_t extra_n,
                                          hipStream_t stream) {
     if constexpr (BF_MODEL != TWO_LORENTZIAN) {
         // Single-peak fits of at most kGuessSortedMaxPoints: eight per wave
@@ -290,15 +290,15 @@ hipError_t launch_ragged_guess<BF_MODEL>(const bf_fit_desc *fits, const uint32_t
             return hipGetLastError();
         }
     }
-    const size_t lds = size_t(m_pad + sort_n) * sizeof(float);
-    if (lds > kGuessLdsBytes) {  // the slice's longest fit does not fit in LDS
+    const size_t lds = size_t(m_pad + extra_n) * sizeof(float);
+    if (m_pad > kGuessWaveMaxPoints || lds > kGuessLdsBytes) {  // by the slice's longest fit
         hipLaunchKernelGGL((guess_ragged_wide_kernel<BF_MODEL>),
                            dim3((count + kWideWaves - 1) / kWideWaves), dim3(64 * kWideWaves), 0,
                            stream, fits, order, count, grids, grid_info, x, y, guess);
         return hipGetLastError();
     }
     hipLaunchKernelGGL((guess_ragged_kernel<BF_MODEL>), dim3(count), dim3(64), lds, stream, fits,
-                       order, count, grids, grid_info, x, y, guess, m_pad, sort_n);
+                       order, count, grids, grid_info, x, y, guess, m_pad, extra_n);
     return hipGetLastError();
 }
 

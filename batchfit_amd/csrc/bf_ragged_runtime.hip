@@ -47,12 +47,12 @@ hipError_t ragged_lm(uint32_t model, int cls, bool mixed, const RaggedArgs &a, u
 hipError_t ragged_guess(uint32_t model, const bf_fit_desc *fits, const uint32_t *order,
                         uint32_t count, const bf_grid_desc *grids, const GridInfo *info,
                         const float *x, const float *y, float *guess, uint32_t m_pad,
-                        uint32_t sort_n, hipStream_t s) {
+                        uint32_t extra_n, hipStream_t s) {
     switch (model) {
-    case 0: return launch_ragged_guess<0>(fits, order, count, grids, info, x, y, guess, m_pad, sort_n, s);
-    case 1: return launch_ragged_guess<1>(fits, order, count, grids, info, x, y, guess, m_pad, sort_n, s);
-    case 2: return launch_ragged_guess<2>(fits, order, count, grids, info, x, y, guess, m_pad, sort_n, s);
-    case 3: return launch_ragged_guess<3>(fits, order, count, grids, info, x, y, guess, m_pad, sort_n, s);
+    case 0: return launch_ragged_guess<0>(fits, order, count, grids, info, x, y, guess, m_pad, extra_n, s);
+    case 1: return launch_ragged_guess<1>(fits, order, count, grids, info, x, y, guess, m_pad, extra_n, s);
+    case 2: return launch_ragged_guess<2>(fits, order, count, grids, info, x, y, guess, m_pad, extra_n, s);
+    case 3: return launch_ragged_guess<3>(fits, order, count, grids, info, x, y, guess, m_pad, extra_n, s);
     }
     return hipErrorInvalidValue;
 }
@@ -225,9 +225,9 @@ hipError_t enqueue_run(const RaggedLayout &layout, size_t i, const RaggedDevice 
     const RaggedRun &r = layout.runs[i];
     hipError_t e;
     for (const GuessSlice &g : r.guess_slices) {
-        const uint32_t sort_n = guess_extra_floats(int(r.model), g.max_points);
+        const uint32_t extra_n = guess_extra_floats(int(r.model), g.max_points);
         if ((e = ragged_guess(r.model, dev.fits, dev.order + r.offset + g.offset, g.count,
-                              dev.grids, dev.info, io.x, io.y, dev.guess, g.max_points, sort_n,
+                              dev.grids, dev.info, io.x, io.y, dev.guess, g.max_points, extra_n,
                               s)) != hipSuccess)
             return e;
     }

@@ -313,18 +313,19 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
 
     const float *init = p0;
     if (!init) {
-        const uint32_t sort_n = guess_extra_floats(int(model), m);
-        // The single-peak models' guess streams the row; the two-peak guess
-        // holds the signal in LDS up to kGuessLdsBytes and reads it from
-        // memory beyond (no limit on M either way).
-        const size_t lds = size_t(m + sort_n) * sizeof(float);
+        const uint32_t extra_n = guess_extra_floats(int(model), m);
+        // The single-peak models' guess streams the row (the one-wave kernel
+        // for edge windows of 129–230 values); the two-peak guess holds the
+        // signal in LDS up to kGuessWaveMaxPoints and reads it from memory
+        // beyond (no limit on M either way).
+        const size_t lds = size_t(m + extra_n) * sizeof(float);
         float *guess = nullptr;
         if (fixed)
             guess = reinterpret_cast<float *>(fixed + 512);  // every row written by the guess
         else
             BF_TRY(temp.get(reinterpret_cast<void **>(&guess), size_t(n) * P * sizeof(float)),
                    "hipMallocAsync(guess)");
-        GuessArgs ga{x, y, (long long)m, 1, grid, n, m, m, sort_n, guess, nullptr, nullptr};
+        GuessArgs ga{x, y, (long long)m, 1, grid, n, m, m, extra_n, guess, nullptr, nullptr};
         BF_TRY(launch_guess(model, ga, lds, stream), "initial guess launch");
         init = guess;
         tm.phase[1] = true;
@@ -623,11 +624,14 @@ int bf_hip_guess_form(uint32_t model, uint32_t form, uint32_t n, uint32_t m, con
         return BF_OK;
     if (!x || !y || !params || !baseline || !gstatus)
         return fail(err, errlen, BF_INVALID_ARGUMENT, "every array must be non-null");
-    const uint32_t sort_n = guess_extra_floats(int(model), m);
-    const size_t lds = size_t(m + sort_n) * sizeof(float);
-    if (form == BF_GUESS_WAVE && lds > kGuessLdsBytes)
-        return fail(err, errlen, BF_INVALID_ARGUMENT,
-                    "the one-wave guess holds M up to 12,288 points in LDS");
+    const uint32_t extra_n = guess_extra_floats(int(model), m);
+    const size_t lds = size_t(m + extra_n) * sizeof(float);
+    if (form == BF_GUESS_WAVE && lds > kGuessLdsBytes) {
+        char msg[96];
+        std::snprintf(msg, sizeof msg, "the one-wave guess of this model holds M up to %u points in LDS",
+                      guess_lds_max_points(int(model)));
+        return fail(err, errlen, BF_INVALID_ARGUMENT, msg);
+    }
     Runtime &rt = Runtime::get();
     std::lock_guard<std::mutex> lock(rt.host_calls);
     hipStream_t s = rt.stream;
@@ -646,7 +650,7 @@ int bf_hip_guess_form(uint32_t model, uint32_t form, uint32_t n, uint32_t m, con
     BF_TRY(hipMemcpyAsync(dy, y, size_t(n) * m * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     hipLaunchKernelGGL(grid_kernel, dim3(1), dim3(256), 0, s, dx, m, grid, nullptr);
     BF_TRY(hipGetLastError(), "grid prologue launch");
-    GuessArgs ga{dx, dy, (long long)m, 1, grid, n, m, m, sort_n, dp, db, dg};
+    GuessArgs ga{dx, dy, (long long)m, 1, grid, n, m, m, extra_n, dp, db, dg};
     BF_TRY(launch_guess(model, ga, lds, s, form), "initial guess launch");
     BF_TRY(hipMemcpyAsync(params, dp, size_t(n) * P * 4, hipMemcpyDeviceToHost, s), "copy");
     BF_TRY(hipMemcpyAsync(baseline, db, size_t(n) * 4, hipMemcpyDeviceToHost, s), "copy");

@@ -293,7 +293,7 @@ int bf_hip_guess(uint32_t model, uint32_t n, uint32_t m, const float *x, const f
 /* Guess kernels (every form gives the same bits): */
 enum bf_guess_form {
     BF_GUESS_AUTO = 0, /* the fastest that applies (bf_hip_guess) */
-    BF_GUESS_WAVE = 1, /* one wave per fit, the signal staged in LDS (M <= 12,288) */
+    BF_GUESS_WAVE = 1, /* one wave per fit, the signal staged in LDS (M <= 16,384; 15,886 two-peak) */
     BF_GUESS_WIDE = 2  /* one wave per fit, the signal read from memory (any M) */
 };
 /* bf_hip_guess by a chosen kernel, for tests that hold the forms to each

@@ -131,7 +131,7 @@ def test_guess_bitwise_sets(hip, oracle_port, model, m, n):
 @pytest.mark.parametrize("m", [1500, 2600, 4096, 6000])
 def test_guess_bitwise_long_windows(hip, oracle_port, model, m):
     """Edge windows of 150–600 points: medians by register selection (up to
-    1024 values) and by the LDS sort beyond. Bitwise on noisy data; on a
+    1024 values) and by value search beyond. Bitwise on noisy data; on a
     quantised copy, where many equal values (zeros of both signs among them)
     meet at the median, equal in value: which of two equal elements the
     reference's nth_element leaves in the middle is unspecified, so the sign
@@ -206,22 +206,27 @@ def test_guess_wide_equals_wave_and_oracle(hip, oracle_port, model, m):
     with np.errstate(over="ignore"):  # the overflow rows quantise to ±inf
         q = (np.round(rows * 50) / 50).astype(np.float32)
     ref = oracle_port.guess(xc, q, model, threads=8)
-    for a, b in zip(device_guess(xc, q, model, GUESS_WIDE), ref):
-        assert np.array_equal(a, b, equal_nan=True), (model, m, "quantised")
+    for form in (GUESS_WIDE, GUESS_WAVE):
+        for a, b in zip(device_guess(xc, q, model, form), ref):
+            assert np.array_equal(a, b, equal_nan=True), (model, m, form, "quantised")
 
 
 @pytest.mark.parametrize("m", [12289, 16384, 70000])
 def test_two_peak_guess_long_spectra(hip, oracle_port, m):
-    """Doublets beyond the one-wave guess's LDS take the wide guess: bit for
+    """Long doublets take the wide guess (the automatic choice beyond 5,120
+    points); up to 15,886 points the one-wave guess also holds them, with
+    its edge levels searched by value over the staged signal. Both bit for
     bit the oracle's guess (initial_guess.hpp keeps its scratch on the heap,
-    :143), at 70,000 points with suffix minima over pairs of 64-point steps;
-    fits from it within the FP32 gate."""
+    :143), at 70,000 points with step minima over runs of four 64-point
+    steps; fits from it within the FP32 gate."""
     x, y, _, bounds, _ = datasets.comparison_set("two_lorentzian", 6, m)
     x, y = x.astype(np.float32), y.astype(np.float32)
     xw, yw = prominence_rows(m, 12, np.random.default_rng(m))
     for xs, ys in ((x, y), (xw, yw)):
         ref = oracle_port.guess(xs, ys, "two_lorentzian", threads=8)
         assert bitwise(device_guess(xs, ys, "two_lorentzian"), ref), m
+        if m <= 15886:
+            assert bitwise(device_guess(xs, ys, "two_lorentzian", GUESS_WAVE), ref), (m, "wave")
     assert np.all(oracle_port.guess(x, y, "two_lorentzian", threads=8)[2] == 0)
     fit = hip.fit(x, y, "two_lorentzian", None, bounds, "hip")
     assert_fp32_gate(fit, oracle_port.fit(x, y, "two_lorentzian", None, bounds, threads=8),
@@ -616,13 +621,16 @@ def test_guess_long_spectra(hip, oracle_port, model, m):
 
 
 def test_guess_form_errors(hip):
-    """The one-wave form holds at most 12,288 points in LDS; unknown forms are
-    argument errors."""
+    """The one-wave form holds at most 15,886 two-peak points in LDS (16,384
+    single-peak); unknown forms are argument errors."""
     from batchfit_amd import _native
-    x = np.linspace(-5, 5, 12289).astype(np.float32)
+    x = np.linspace(-5, 5, 15887).astype(np.float32)
     y = np.ones((2, x.size), np.float32)
-    with pytest.raises(ValueError, match="12,288"):
+    with pytest.raises(ValueError, match="holds M up to 15886 points"):
         device_guess(x, y, "two_lorentzian", GUESS_WAVE)
+    x1 = np.linspace(-5, 5, 16385).astype(np.float32)
+    with pytest.raises(ValueError, match="holds M up to 16384 points"):
+        device_guess(x1, np.ones((2, x1.size), np.float32), "lorentzian", GUESS_WAVE)
     with pytest.raises(ValueError, match="unknown guess form"):
         device_guess(x, y, "two_lorentzian", 3)
 

@@ -108,13 +108,13 @@ def test_fixed_baseline_doublets_equal_full_form(hip, oracle_port, shift):
 
 
 def test_ragged_long_spectra_from_guess(hip):
-    """Guessed fits beyond the one-wave guess's LDS (M > 12,288 points) take
+    """Guessed fits beyond the one-wave guess's LDS (M > 16,384 points) take
     the wide guess in a ragged batch: the same bytes as the uniform entry
     (whose single-peak guess is the streamed group form), beside short fits."""
     from batchfit_amd.ragged import fit_ragged
     batches = []
     for model, m, n in (("two_lorentzian", 16384, 5), ("lorentzian", 20000, 4),
-                        ("gaussian", 13000, 3), ("pseudo_voigt", 300, 30)):
+                        ("gaussian", 17000, 3), ("pseudo_voigt", 300, 30)):
         x, y, p0, bounds, _ = datasets.comparison_set(model, n, m)
         batches.append((x.astype(np.float32), y.astype(np.float32), model, None, bounds))
     together = fit_ragged(batches)
