@@ -36,6 +36,12 @@ struct Runtime {
     std::string why = "no HIP device";
     hipStream_t stream = nullptr;
     hipStream_t side[2] = {nullptr, nullptr};  // the host-array entry's chunk streams
+    // Pinned staging slots of the host-array entry (hipHostMalloc, kept for
+    // the process: pinning memory costs more than a chunk's copy).
+    char *staging[2] = {nullptr, nullptr};
+    size_t staging_bytes = 0;
+    char *staging_out[2] = {nullptr, nullptr};  // a chunk's results, device → host
+    size_t staging_out_bytes = 0;
     std::mutex host_calls;  // serialises the synchronous host-array entry (bindings.cpp:13)
 
     Runtime() {

@@ -15,8 +15,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 namespace bfhip {
 
@@ -395,6 +399,82 @@ int fit_device(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
 
 using namespace bfhip;
 
+namespace {
+
+// Host threads that copy one buffer into another together, each its own
+// piece: pageable rows into a pinned staging slot, at several times the rate
+// of one thread, so the DMA from the slot is what sets the pace.
+class CopyCrew {
+  public:
+    explicit CopyCrew(unsigned threads) : threads_(threads) {
+        for (unsigned k = 0; k < threads_; ++k)
+            pool_.emplace_back([this, k] { run(k); });
+    }
+    ~CopyCrew() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        work_.notify_all();
+        for (auto &t : pool_)
+            t.join();
+    }
+    // dst[0..bytes) = src[0..bytes); returns when every piece is copied.
+    void copy(void *dst, const void *src, size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            dst_ = static_cast<char *>(dst);
+            src_ = static_cast<const char *>(src);
+            bytes_ = bytes;
+            left_ = threads_;
+            ++generation_;
+        }
+        work_.notify_all();
+        std::unique_lock<std::mutex> lock(mu_);
+        done_.wait(lock, [&] { return left_ == 0; });
+    }
+
+  private:
+    void run(unsigned k) {
+        uint64_t seen = 0;
+        for (;;) {
+            char *dst;
+            const char *src;
+            size_t bytes;
+            {
+                std::unique_lock<std::mutex> lock(mu_);
+                work_.wait(lock, [&] { return stop_ || generation_ != seen; });
+                if (stop_)
+                    return;
+                seen = generation_;
+                dst = dst_;
+                src = src_;
+                bytes = bytes_;
+            }
+            // Pieces of whole 4 KiB pages.
+            const size_t step = ((bytes + threads_ - 1) / threads_ + 4095) & ~size_t(4095);
+            const size_t a = std::min(bytes, size_t(k) * step), b = std::min(bytes, a + step);
+            if (a < b)
+                std::memcpy(dst + a, src + a, b - a);
+            std::lock_guard<std::mutex> lock(mu_);
+            if (--left_ == 0)
+                done_.notify_one();
+        }
+    }
+    const unsigned threads_;
+    std::vector<std::thread> pool_;
+    std::mutex mu_;
+    std::condition_variable work_, done_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0;
+    unsigned left_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+};
+
+}  // namespace
+
 extern "C" {
 
 int bf_hip_abi_version(void) { return BF_HIP_ABI_VERSION; }
@@ -520,6 +600,51 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         BF_TRY(scratch.get(reinterpret_cast<void **>(&dslot[k]), slot_bytes),
                "hipMallocAsync(chunk buffers)");
     BF_TRY(hipStreamSynchronize(s), "chunk buffer allocation");  // usable on the side streams
+    // Several chunks: y goes through two pinned staging slots, filled by a
+    // crew of host threads while the DMA engine moves the other slot, so the
+    // copy-in runs at the link's pinned rate instead of the runtime's
+    // single-threaded staging of pageable memory (chunks of at most 256 MiB;
+    // larger rows keep the pageable copies).
+    // A chunk's results come back in one copy into a pinned slot and are
+    // unpacked on the host (a copy to pageable memory costs as much as a
+    // chunk's fit, and there are up to seven per chunk).
+    const size_t y_chunk_bytes = chunk * m * 4, out_chunk_bytes = chunk * out_per_fit;
+    const bool staged = chunks > 1 && y_chunk_bytes <= (size_t(256) << 20);
+    auto ensure = [&](char *(&slots_)[2], size_t &have, size_t need) -> int {
+        if (have >= need)
+            return BF_OK;
+        for (char *&slot : slots_)
+            if (slot) {
+                BF_TRY(hipHostFree(slot), "hipHostFree(staging)");
+                slot = nullptr;
+            }
+        have = 0;
+        for (char *&slot : slots_)
+            BF_TRY(hipHostMalloc(reinterpret_cast<void **>(&slot), need), "hipHostMalloc(staging)");
+        have = need;
+        return BF_OK;
+    };
+    if (staged) {
+        if (const int rc = ensure(rt.staging, rt.staging_bytes, y_chunk_bytes))
+            return rc;
+        if (const int rc = ensure(rt.staging_out, rt.staging_out_bytes, out_chunk_bytes))
+            return rc;
+    }
+    struct Uploads {
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Uploads() {
+            for (hipEvent_t e : ev)
+                if (e)
+                    (void)hipEventDestroy(e);
+        }
+    } uploads;
+    std::unique_ptr<CopyCrew> crew;
+    if (staged) {
+        for (hipEvent_t &e : uploads.ev)
+            BF_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        crew = std::make_unique<CopyCrew>(std::min(8u, hw));
+    }
     // Whatever way this call returns, the side streams finish with the chunk
     // buffers before `scratch` frees them on the library stream.
     struct SideDone {
@@ -553,6 +678,27 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         const Slot v = carve(dslot[c % slots]);
         hipStream_t st = rt.side[c % slots];
         const size_t f0 = c * chunk, nc = std::min(chunk, n - f0);
+        if (staged) {
+            char *h = rt.staging_out[c % slots];
+            BF_TRY(hipMemcpyAsync(h, v.params, out_chunk_bytes, hipMemcpyDeviceToHost, st),
+                   "copy results");
+            BF_TRY(hipStreamSynchronize(st), "results of a chunk");
+            // The slot's arrays as carved on the device: params | chi2 |
+            // status | iterations | cov | stderr | ustatus, `chunk` rows each.
+            const char *hp = h, *hc = hp + chunk * P * 4, *hs = hc + chunk * 4,
+                       *hi = hs + chunk * 4, *hv = hi + chunk * 4;
+            std::memcpy(params + f0 * P, hp, nc * P * 4);
+            std::memcpy(chi2 + f0, hc, nc * 4);
+            std::memcpy(status + f0, hs, nc * 4);
+            std::memcpy(iterations + f0, hi, nc * 4);
+            if (unc) {
+                const char *he = hv + chunk * P * P * 4, *hu = he + chunk * P * 4;
+                std::memcpy(cov + f0 * P * P, hv, nc * P * P * 4);
+                std::memcpy(stderr_ + f0 * P, he, nc * P * 4);
+                std::memcpy(ustatus + f0, hu, nc * 4);
+            }
+            return BF_OK;
+        }
         BF_TRY(hipMemcpyAsync(params + f0 * P, v.params, nc * P * 4, hipMemcpyDeviceToHost, st),
                "copy params");
         BF_TRY(hipMemcpyAsync(chi2 + f0, v.chi2, nc * 4, hipMemcpyDeviceToHost, st), "copy chi2");
@@ -578,8 +724,19 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
         const Slot v = carve(dslot[c % slots]);
         hipStream_t st = rt.side[c % slots];
         const size_t f0 = c * chunk, nc = std::min(chunk, n - f0);
-        BF_TRY(hipMemcpyAsync(v.y, y + f0 * m, nc * m * 4, hipMemcpyHostToDevice, st),
-               "hipMemcpyAsync(y)");
+        if (staged) {
+            // The slot's previous upload (chunk c − 2) has left it.
+            if (c >= slots)
+                BF_TRY(hipEventSynchronize(uploads.ev[c % slots]), "staging slot reuse");
+            char *pinned = rt.staging[c % slots];
+            crew->copy(pinned, y + f0 * m, nc * m * 4);
+            BF_TRY(hipMemcpyAsync(v.y, pinned, nc * m * 4, hipMemcpyHostToDevice, st),
+                   "hipMemcpyAsync(y)");
+            BF_TRY(hipEventRecord(uploads.ev[c % slots], st), "hipEventRecord(upload)");
+        } else {
+            BF_TRY(hipMemcpyAsync(v.y, y + f0 * m, nc * m * 4, hipMemcpyHostToDevice, st),
+                   "hipMemcpyAsync(y)");
+        }
         if (p0)
             BF_TRY(hipMemcpyAsync(v.p0, p0 + f0 * P, nc * P * 4, hipMemcpyHostToDevice, st),
                    "hipMemcpyAsync(p0)");
