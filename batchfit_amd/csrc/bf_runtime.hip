@@ -406,19 +406,18 @@ namespace {
 // of one thread, so the DMA from the slot is what sets the pace.
 class CopyCrew {
   public:
+    // Throws std::system_error if a thread cannot be started (after joining
+    // the ones that were).
     explicit CopyCrew(unsigned threads) : threads_(threads) {
-        for (unsigned k = 0; k < threads_; ++k)
-            pool_.emplace_back([this, k] { run(k); });
-    }
-    ~CopyCrew() {
-        {
-            std::lock_guard<std::mutex> lock(mu_);
-            stop_ = true;
+        try {
+            for (unsigned k = 0; k < threads_; ++k)
+                pool_.emplace_back([this, k] { run(k); });
+        } catch (...) {
+            halt();
+            throw;
         }
-        work_.notify_all();
-        for (auto &t : pool_)
-            t.join();
     }
+    ~CopyCrew() { halt(); }
     // dst[0..bytes) = src[0..bytes); returns when every piece is copied.
     void copy(void *dst, const void *src, size_t bytes) {
         {
@@ -435,6 +434,16 @@ class CopyCrew {
     }
 
   private:
+    void halt() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        work_.notify_all();
+        for (auto &t : pool_)
+            t.join();
+        pool_.clear();
+    }
     void run(unsigned k) {
         uint64_t seen = 0;
         for (;;) {
@@ -609,7 +618,16 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     // unpacked on the host (a copy to pageable memory costs as much as a
     // chunk's fit, and there are up to seven per chunk).
     const size_t y_chunk_bytes = chunk * m * 4, out_chunk_bytes = chunk * out_per_fit;
-    const bool staged = chunks > 1 && y_chunk_bytes <= (size_t(256) << 20);
+    std::unique_ptr<CopyCrew> crew;
+    if (chunks > 1 && y_chunk_bytes <= (size_t(256) << 20)) {
+        try {
+            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+            crew = std::make_unique<CopyCrew>(std::min(8u, hw));
+        } catch (const std::exception &) {
+            crew.reset();  // no threads to be had: the runtime's pageable copies
+        }
+    }
+    const bool staged = crew != nullptr;
     auto ensure = [&](char *(&slots_)[2], size_t &have, size_t need) -> int {
         if (have >= need)
             return BF_OK;
@@ -638,13 +656,9 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
                     (void)hipEventDestroy(e);
         }
     } uploads;
-    std::unique_ptr<CopyCrew> crew;
-    if (staged) {
+    if (staged)
         for (hipEvent_t &e : uploads.ev)
             BF_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        crew = std::make_unique<CopyCrew>(std::min(8u, hw));
-    }
     // Whatever way this call returns, the side streams finish with the chunk
     // buffers before `scratch` frees them on the library stream.
     struct SideDone {
