@@ -646,6 +646,29 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
         wave_suffix_min(sfx, runs, lane);
         float v_best = 0.0f;
         int j_best = 0x7fffffff;
+        // The queue: lane q < queued holds the candidate at qj of height qh.
+        int queued = 0, qj = 0;
+        float qh = 0.0f;
+        auto drain_queue = [&]() {
+            const bool mine = lane < queued;
+            const int state = mine ? local_sides(s, m, qj, qh, t) : 0;
+            const uint64_t settled_yes = __ballot(mine && state == 1);
+            const uint64_t settled_no = __ballot(mine && state == 2);
+            const uint64_t all = queued == 64 ? ~0ull : ((1ull << queued) - 1);
+            for (uint64_t mask = all & ~settled_no; mask; mask &= mask - 1) {
+                const int l = __builtin_ctzll(mask);
+                const float h = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qh), l));
+                if (!(h > v_best))
+                    continue;
+                const int jc = __builtin_amdgcn_readlane(qj, l);
+                if (((settled_yes >> l) & 1) || prominent_wave(s, m, jc, h, t, lane)) {
+                    v_best = h;
+                    j_best = jc;
+                }
+            }
+            queued = 0;
+        };
         Raw3 cur = raw3_in(s, m, 1 + lane);
         float before = s[0];
         for (int base = 1; base + 1 < m; base += 64) {
@@ -668,25 +691,39 @@ __device__ GuessResult guess_tail(const S &s, float *sfx, int sfx_shift, const f
                     cand = !(bins < 3 || (idx >= sep_lo && idx < sep_hi));
                 }
             }
-            // Each candidate first reads the nearest kLocalWalk points of
-            // both sides on its own lane (most are settled there); the rest
-            // are walked by the whole wave.
-            const int state = cand ? local_sides(s, m, j, v, t) : 0;
-            const uint64_t settled_yes = __ballot(cand && state == 1);
-            const uint64_t settled_no = __ballot(cand && state == 2);
-            // This step's candidates in increasing index; one that is not
-            // above the best so far cannot win.
-            for (uint64_t mask = __ballot(cand) & ~settled_no; mask; mask &= mask - 1) {
-                const int l = __builtin_ctzll(mask);
-                const float h = __builtin_bit_cast(
-                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-                if (h > v_best &&
-                    (((settled_yes >> l) & 1) || prominent_wave(s, m, base + l, h, t, lane))) {
-                    v_best = h;
-                    j_best = base + l;
+            // Candidates join a queue, one per lane in index order, which is
+            // tested when full and at the end: each queued candidate first
+            // reads the nearest kLocalWalk points of both sides on its own
+            // lane (most are settled there), the rest are walked by the
+            // whole wave. Queued candidates passed `v > v_best` against the
+            // best when they joined; they are taken in index order and must
+            // beat the best at their turn, so the outcome is that of testing
+            // each step's candidates at once, while the per-lane tests run
+            // with every lane busy instead of a few.
+            const uint64_t cm = __ballot(cand);
+            if (cm) {
+                const int nc = __popcll(cm);
+                if (queued + nc > 64) {
+                    drain_queue();
                 }
+                // One forward permute moves the step's candidates, in lane
+                // order, to lanes queued … queued + nc − 1 (the other lanes
+                // fill the remaining places, so every lane has one sender).
+                const int k = lane - queued;
+                const int below = __builtin_amdgcn_mbcnt_hi(
+                    uint32_t(cm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(cm), 0));
+                const int dest = cand ? queued + below : (queued + nc + lane - below) & 63;
+                const int sj = __builtin_amdgcn_ds_permute(dest * 4, j);
+                const float sh =
+                    __int_as_float(__builtin_amdgcn_ds_permute(dest * 4, __float_as_int(v)));
+                if (k >= 0 && k < nc) {
+                    qj = sj;
+                    qh = sh;
+                }
+                queued += nc;
             }
         }
+        drain_queue();
         int second = j_best == 0x7fffffff ? -1 : raw_peak_dev(s, j_best);
         if (second < 0)
             return finish();
@@ -855,3 +892,6 @@ template <int MODEL> __global__ void __launch_bounds__(64) guess_kernel(GuessArg
 #pragma clang fp contract(fast)
 
 }  // namespace bfhip
+
+// stream_levels (declared above, used by guess_core) is defined there.
+#include "bf_guess_group.hpp"
