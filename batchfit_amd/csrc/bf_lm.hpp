@@ -571,10 +571,15 @@ template <int MODEL> __device__ __forceinline__ int32_t resolved_status(const fl
 // information pass sums only the six free parameters' rows; the baseline
 // rows are replaced by the identity's below in either form, and on such a
 // grid their sums are finite whenever the free rows' are (Cauchy–Schwarz),
-// so no finiteness test changes.
-template <int MODEL, int G, int R, typename S, bool FIXB = false>
+// so no finiteness test changes. CLDS (groups of at most 64 lanes): the
+// unsymmetrised covariance is kept in cl (P·P values, the group's own, in
+// LDS) instead of in every lane's registers (for P = 8 in FP64 that is 128
+// registers, which would hold the kernel to one wave per SIMD), and its
+// columns are solved by different lanes instead of by every lane in turn.
+template <int MODEL, int G, int R, typename S, bool FIXB = false, bool CLDS = false>
 __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, stream_chunk(MODEL)> &pts, uint32_t fit,
-                                           int gl, const float *p, int32_t status, float chi2) {
+                                           int gl, const float *p, int32_t status, float chi2,
+                                           S *cl = nullptr) {
     constexpr int P = Params<MODEL>::P, TRI = Params<MODEL>::TRI;
     const float nan = qnan_f();
     float *cov = a.cov + size_t(fit) * P * P;
@@ -629,6 +634,10 @@ __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, 
         return fail(US_INVALID);
     // The factor does not depend on the right-hand side: factor once, then
     // solve each unit column (the reference refactors per column; same values).
+    S d[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        d[i] = sqrt(info[tri(i, i)]);
     S factor[TRI];
 #pragma unroll
     for (int t = 0; t < TRI; ++t)
@@ -640,16 +649,36 @@ __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, 
     // Factor with a zero right-hand side to obtain the scaled factor.
     if (!chol_solve<S, P>(factor, unit, true))
         return fail(US_SINGULAR);
-    S d[P];
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-        d[i] = sqrt(info[tri(i, i)]);
+    for (int i = 0; i < P; ++i)
         if (!fixed[i] && factor[tri(i, i)] * factor[tri(i, i)] <= S(64) * S(1.1920928955078125e-7))
             return fail(US_SINGULAR);
-    }
-    S c[P * P];
+    // c[i·P + col]: in cl (lane 0's copy) or in registers; its diagonal also
+    // in registers, for the sign test.
+    S creg[CLDS ? 1 : P * P];
+    S cdiag[CLDS ? 1 : P];
+    uint32_t fixed_mask = 0;
 #pragma unroll
-    for (int col = 0; col < P; ++col) {
+    for (int k = 0; k < P; ++k)
+        fixed_mask |= uint32_t(fixed[k]) << k;
+    auto cset = [&](int i, int col, S v) {
+        if constexpr (CLDS) {
+            cl[i * P + col] = v;  // by the lane that solves the column
+        } else {
+            creg[i * P + col] = v;
+            if (i == col)
+                cdiag[i] = v;
+        }
+    };
+    auto cget = [&](int i, int j) {
+        if constexpr (CLDS)
+            return cl[i * P + j];
+        else
+            return creg[i * P + j];
+    };
+    // Column col of the covariance: 0, or the uncertainty status to fail
+    // with.
+    auto column = [&](int col) -> int32_t {
         S b[P];
 #pragma unroll
         for (int i = 0; i < P; ++i)
@@ -678,34 +707,77 @@ __device__ __forceinline__ void covariance(const FitArgs &a, const Points<G, R, 
             b[i] /= d[i];
             ok = ok && finite_s(b[i]);
         }
-        if (fixed[col]) {
+        if ((fixed_mask >> col) & 1) {
 #pragma unroll
             for (int i = 0; i < P; ++i)
-                c[i * P + col] = S(0);
-            continue;
+                cset(i, col, S(0));
+            return 0;
         }
         if (!ok)
-            return fail(US_SINGULAR);
+            return US_SINGULAR;
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            c[i * P + col] = fixed[i] ? S(0) : b[i] * variance;
-            if (!finite_s(c[i * P + col]))
-                return fail(US_INVALID);
+            const S v = fixed[i] ? S(0) : b[i] * variance;
+            cset(i, col, v);
+            if (!finite_s(v))
+                return US_INVALID;
         }
+        return 0;
+    };
+    if constexpr (CLDS) {
+        // The columns are independent: lane gl of the group solves columns
+        // gl, gl + G, … into cl (the registers hold the factor and one
+        // column). The first failing column in index order decides the
+        // status, as when the columns are taken in turn; then lane 0 tests
+        // and writes the result.
+        int first_fail = 0x7fffffff;  // 8·column + status
+#pragma unroll 1
+        for (int col = gl; col < P; col += G)
+            if (const int32_t us = column(col)) {
+                first_fail = 8 * col + us;
+                break;
+            }
+        first_fail = group_min_i<G>(first_fail);
+        if (first_fail != 0x7fffffff)
+            return fail(first_fail & 7);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (gl == 0) {
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                if (cl[i * P + i] < S(0))
+                    return fail(US_INVALID);
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const S sym = i == j ? cl[i * P + i] : cl[i * P + j] / S(2) + cl[j * P + i] / S(2);
+                    cov[i * P + j] = float(sym);
+                }
+                se[i] = float(sqrt(cl[i * P + i]));
+            }
+            a.ustatus[fit] = US_AVAILABLE;
+        }
+        return;
     }
 #pragma unroll
+    for (int col = 0; col < P; ++col)
+        if (const int32_t us = column(col))
+            return fail(us);
+#pragma unroll
     for (int i = 0; i < P; ++i)
-        if (c[i * P + i] < S(0))
+        if (cdiag[i] < S(0))
             return fail(US_INVALID);
     if (gl == 0) {
 #pragma unroll
         for (int i = 0; i < P; ++i) {
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const S sym = i == j ? c[i * P + i] : c[i * P + j] / S(2) + c[j * P + i] / S(2);
+                const S sym = i == j ? cdiag[i] : cget(i, j) / S(2) + cget(j, i) / S(2);
                 cov[i * P + j] = float(sym);
             }
-            se[i] = float(sqrt(c[i * P + i]));
+            se[i] = float(sqrt(cdiag[i]));
         }
         a.ustatus[fit] = US_AVAILABLE;
     }
@@ -1252,7 +1324,9 @@ __global__ void __launch_bounds__(256) lm_cov_kernel(FitArgs a) {
         p[k] = a.params[size_t(fit) * P + k];
     Points<G, 0, stream_chunk(MODEL)> pts;
     pts.load(a.x, a.y + fit * a.ys_fit, a.ys_pt, gl, int(a.m));
-    covariance<MODEL, G, 0, S, FIXB>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit]);
+    __shared__ S cl[256 / G][P * P];
+    covariance<MODEL, G, 0, S, FIXB, true>(a, pts, fit, gl, p, a.status[fit], a.chi2[fit],
+                                           cl[threadIdx.x / G]);
 }
 
 // Workgroup per fit (G = 128 or 256 lanes, 2 or 4 waves): every workgroup

@@ -237,7 +237,9 @@ __global__ void __launch_bounds__(256) lm_ragged_cov_kernel(RaggedArgs r) {
         p[k] = a.params[k];
     Points<G, 0, stream_chunk(MODEL)> pts;
     pts.load(a.x, a.y, 1, gl, int(a.m));
-    covariance<MODEL, G, 0, S, FIXB>(a, pts, 0, gl, p, a.status[0], a.chi2[0]);
+    __shared__ S cl[256 / G][P * P];
+    covariance<MODEL, G, 0, S, FIXB, true>(a, pts, 0, gl, p, a.status[0], a.chi2[0],
+                                           cl[threadIdx.x / G]);
 }
 
 // Data-only guess of one class of a ragged batch, one wave per fit; LDS sized
