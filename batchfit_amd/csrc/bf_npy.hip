@@ -170,6 +170,20 @@ struct PipeSlot {
 
 constexpr size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Ring slots of finished calls, kept for the next call: pinning host memory
+// and touching it the first time cost more than streaming the chunks through
+// it. Held until the process ends (never freed, so nothing runs at exit);
+// a call takes slots out and gives them back, so concurrent calls never share
+// one.
+struct SlotPool {
+    std::mutex mu;
+    std::vector<std::unique_ptr<PipeSlot>> idle;
+};
+SlotPool &slot_pool() {
+    static SlotPool *pool = new SlotPool;
+    return *pool;
+}
+
 void check(hipError_t e, const char *what) {
     if (e != hipSuccess)
         runtime(std::string(what) + " failed: " + hipGetErrorString(e));
@@ -276,10 +290,44 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
         const uint64_t in_bytes = chunk * m * yf.h.item;
         const uint64_t p0_in = pf ? chunk * P * pf->h.item : 0;
         std::vector<std::unique_ptr<PipeSlot>> slots;
+        // However this call ends, its slots go back to the pool once their
+        // streams are idle.
+        struct GiveBack {
+            std::vector<std::unique_ptr<PipeSlot>> &slots;
+            ~GiveBack() {
+                SlotPool &pool = slot_pool();
+                for (auto &s : slots)
+                    if (s->stream)
+                        (void)hipStreamSynchronize(s->stream);
+                std::lock_guard<std::mutex> lock(pool.mu);
+                for (auto &s : slots) {
+                    s->busy = false;
+                    pool.idle.push_back(std::move(s));
+                }
+            }
+        } give_back{slots};
+        size_t ring_bytes = 0;
         for (uint32_t k = 0; k < opt.in_flight; ++k) {
-            auto s = std::make_unique<PipeSlot>();
-            check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
-            check(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "hipEventCreate");
+            std::unique_ptr<PipeSlot> s;
+            {
+                SlotPool &pool = slot_pool();
+                std::lock_guard<std::mutex> lock(pool.mu);
+                if (!pool.idle.empty()) {
+                    s = std::move(pool.idle.back());
+                    pool.idle.pop_back();
+                }
+            }
+            if (!s)
+                s = std::make_unique<PipeSlot>();
+            slots.push_back(std::move(s));  // given back even if the setup below fails
+            PipeSlot *sp = slots.back().get();
+            if (!sp->stream)
+                check(hipStreamCreateWithFlags(&sp->stream, hipStreamNonBlocking),
+                      "hipStreamCreate");
+            if (!sp->done)
+                check(hipEventCreateWithFlags(&sp->done, hipEventDisableTiming), "hipEventCreate");
+            sp->busy = false;
+            sp->first = sp->count = 0;
             size_t off = 0;
             auto carve = [&](size_t b) {
                 const size_t at = off;
@@ -288,15 +336,23 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
             };
             carve(in_bytes);  // staged y window (as stored)
             carve(p0_in);     // staged p0 window (as stored)
-            s->out_params = carve(chunk * P * 4);
-            s->out_status = carve(chunk * 4);
-            s->out_iter = carve(chunk * 4);
-            s->out_chi2 = carve(chunk * 4);
-            s->out_cov = unc ? carve(chunk * P * P * 4) : 0;
-            s->out_se = unc ? carve(chunk * P * 4) : 0;
-            s->out_us = unc ? carve(chunk * 4) : 0;
-            s->pinned_bytes = off;
-            check(hipHostMalloc(reinterpret_cast<void **>(&s->pinned), off), "hipHostMalloc(ring)");
+            sp->out_params = carve(chunk * P * 4);
+            sp->out_status = carve(chunk * 4);
+            sp->out_iter = carve(chunk * 4);
+            sp->out_chi2 = carve(chunk * 4);
+            sp->out_cov = unc ? carve(chunk * P * P * 4) : 0;
+            sp->out_se = unc ? carve(chunk * P * 4) : 0;
+            sp->out_us = unc ? carve(chunk * 4) : 0;
+            ring_bytes += off;
+            if (sp->pinned_bytes < off) {
+                if (sp->pinned)
+                    check(hipHostFree(sp->pinned), "hipHostFree(ring)");
+                sp->pinned = nullptr;
+                sp->pinned_bytes = 0;
+                check(hipHostMalloc(reinterpret_cast<void **>(&sp->pinned), off),
+                      "hipHostMalloc(ring)");
+                sp->pinned_bytes = off;
+            }
             off = 0;
             carve(in_bytes);                                   // raw y (float64 windows)
             carve(yf.h.item == 8 ? chunk * m * 4 : 0);         // float32 y after narrowing
@@ -304,13 +360,16 @@ int bf_hip_fit_npy(uint32_t model, const char *x_path, const char *y_path, const
             carve(pf && pf->h.item == 8 ? chunk * P * 4 : 0);  // float32 p0
             carve(chunk * P * 4 + chunk * 12);                 // params, status, iter, chi2
             carve(unc ? chunk * (P * P + P + 1) * 4 : 0);
-            s->device_bytes = off;
-            check(hipMalloc(reinterpret_cast<void **>(&s->device), off), "hipMalloc(ring)");
-            slots.push_back(std::move(s));
+            if (sp->device_bytes < off) {
+                if (sp->device)
+                    check(hipFree(sp->device), "hipFree(ring)");
+                sp->device = nullptr;
+                sp->device_bytes = 0;
+                check(hipMalloc(reinterpret_cast<void **>(&sp->device), off), "hipMalloc(ring)");
+                sp->device_bytes = off;
+            }
         }
-        stats.shared_payload_bytes = 0;
-        for (auto &s : slots)
-            stats.shared_payload_bytes += s->pinned_bytes;
+        stats.shared_payload_bytes = ring_bytes;
 
         std::mutex mu;
         std::condition_variable cv;

@@ -535,7 +535,9 @@ def bench_cube(S, args):
     dx.free()
     dy.free()
     bounds = ([0, -10, 1e-4, 0], [INF, 10, 10, 1])
-    fit_npy(xpath, ypath, "pseudo_voigt", None, bounds, chunk_fits=4096)  # warm-up
+    # Warm-up with the timed call's chunking: loads the kernels and pins and
+    # first touches the ring, which the library keeps for later calls.
+    fit_npy(xpath, ypath, "pseudo_voigt", None, bounds)
     t0 = time.perf_counter()
     result, stats = fit_npy(xpath, ypath, "pseudo_voigt", None, bounds)
     elapsed = time.perf_counter() - t0
