@@ -627,27 +627,29 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
             crew.reset();  // no threads to be had: the runtime's pageable copies
         }
     }
-    const bool staged = crew != nullptr;
-    auto ensure = [&](char *(&slots_)[2], size_t &have, size_t need) -> int {
+    // Pinned slots of at least `need` bytes (kept for later calls); false if
+    // they cannot be had, and then the runtime's pageable copies serve.
+    auto ensure = [&](char *(&slots_)[2], size_t &have, size_t need) -> bool {
         if (have >= need)
-            return BF_OK;
+            return true;
         for (char *&slot : slots_)
             if (slot) {
-                BF_TRY(hipHostFree(slot), "hipHostFree(staging)");
+                if (hipHostFree(slot) != hipSuccess)
+                    return false;
                 slot = nullptr;
             }
         have = 0;
         for (char *&slot : slots_)
-            BF_TRY(hipHostMalloc(reinterpret_cast<void **>(&slot), need), "hipHostMalloc(staging)");
+            if (hipHostMalloc(reinterpret_cast<void **>(&slot), need) != hipSuccess) {
+                slot = nullptr;
+                (void)hipGetLastError();
+                return false;
+            }
         have = need;
-        return BF_OK;
+        return true;
     };
-    if (staged) {
-        if (const int rc = ensure(rt.staging, rt.staging_bytes, y_chunk_bytes))
-            return rc;
-        if (const int rc = ensure(rt.staging_out, rt.staging_out_bytes, out_chunk_bytes))
-            return rc;
-    }
+    const bool staged = crew && ensure(rt.staging, rt.staging_bytes, y_chunk_bytes) &&
+                        ensure(rt.staging_out, rt.staging_out_bytes, out_chunk_bytes);
     struct Uploads {
         hipEvent_t ev[2] = {nullptr, nullptr};
         ~Uploads() {
