@@ -632,13 +632,13 @@ int bf_hip_fit(uint32_t model, const bf_config *cfg, const bf_launch *launch, co
     auto ensure = [&](char *(&slots_)[2], size_t &have, size_t need) -> bool {
         if (have >= need)
             return true;
+        have = 0;  // before any slot is freed: a failure below leaves no slot counted
         for (char *&slot : slots_)
             if (slot) {
                 if (hipHostFree(slot) != hipSuccess)
                     return false;
                 slot = nullptr;
             }
-        have = 0;
         for (char *&slot : slots_)
             if (hipHostMalloc(reinterpret_cast<void **>(&slot), need) != hipSuccess) {
                 slot = nullptr;
